@@ -1,0 +1,934 @@
+// Host side of the MI355X coupled-cluster amplitude engine: IR loader,
+// planner (offset tables for every block contraction), resident device
+// state, and the C ABI declared in include/ccq_hip.h.
+//
+// The contraction program is produced offline by ccq_amd/gen (spin-orbital
+// equations -> spin-integrated block contractions) and embedded as text.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ccq_hip.h"
+#include "kernels.h"
+
+#include "ir_ccsd_rhf.inc"
+#include "ir_ccsd_uhf.inc"
+
+namespace {
+
+struct Fail : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      throw Fail(std::string(#x) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+constexpr int kPartialBlocks = 240;
+
+// ---------------------------------------------------------------------------
+// device buffers
+
+struct DevBuf {
+  double* p = nullptr;
+  int64_t n = 0;
+  void alloc(int64_t count) {
+    n = count;
+    if (count > 0) HIPCHK(hipMalloc(&p, sizeof(double) * count));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// tensors and ops of the loaded program
+
+enum class TKind { Inter, View, Amp, Resid };
+
+struct Tensor {
+  std::string id;
+  TKind kind;
+  int rank = 0;
+  int64_t dims[8] = {0};
+  int64_t strides[8] = {0};
+  double* ptr = nullptr;
+  std::string base;          // View: base array; Amp/Resid: block name
+  std::string spec[8];       // View: per-index "oa", "vb", ...
+};
+
+struct Op {
+  int out = -1, a = -1, b = -1;
+  std::string lc, la, lb;
+  double coef = 1.0;
+  bool assign = false;
+  bool gemm = false;
+  int64_t M = 1, N = 1, K = 1;
+  int64_t t_rowA = 0, t_kA = 0, t_kB = 0, t_colB = 0, t_rowC = 0, t_colC = 0;
+  int a_kfast = 1, b_kfast = 1;
+  double flops = 0, bytes = 0;
+  std::string cls;           // profile class
+};
+
+struct UpdateStep {
+  std::string block;
+  int resid = -1;
+  int nimg = 0;
+  std::vector<int> perms;    // nimg x 8
+  std::vector<double> signs;
+  int dev_perm_off = 0;      // offset into the int table
+  int dev_sign_off = 0;
+};
+
+struct Section {
+  std::string name;
+  enum { Compute, Update, Copy } kind = Compute;
+  std::vector<Op> ops;
+  UpdateStep upd;
+  std::string dst, src;
+};
+
+struct ProfEntry {
+  int64_t calls = 0;
+  double ms = 0, flops = 0, bytes = 0;
+};
+
+}  // namespace
+
+struct ccq_hip_ctx {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // sizes
+  int64_t froz = 0, occ_a = 0, occ_b = 0, orbs = 0;
+  int64_t oa = 0, ob = 0, va = 0, vb = 0, nc = 0;
+  int level = 2;
+  bool rhf = false;
+  double shift = 0.0;
+  int diis = 5;
+  // amplitude layout (ccq t_vec)
+  std::map<std::string, int64_t> blk_off, blk_size;
+  std::map<std::string, std::vector<int64_t>> blk_dims;
+  int64_t t_size = 0;        // ccq's pos(10)-1
+  int64_t dev_t = 0;         // doubles of t_vec kept on the device
+  int64_t vec_size = 0;      // DIIS vector length
+  // device state
+  DevBuf Vaa, Vab, Vbb, Fa, Fb, fdiag_a, fdiag_b, tvec, ring, partial;
+  double* vbb_ptr = nullptr;   // Vbb, or Vaa when ccq's v_bb equals v_aa
+  std::map<std::string, std::unique_ptr<DevBuf>> resid_buf;
+  std::vector<std::unique_ptr<DevBuf>> inter_buf;
+  int64_t* tables = nullptr;
+  int* itab = nullptr;
+  double* dtab = nullptr;
+  // program
+  std::vector<Tensor> tensors;
+  std::map<std::string, int> tindex;
+  std::vector<Section> sections;
+  bool ints_ready = false;
+  // profiling
+  bool profile = false;
+  std::map<std::string, ProfEntry> prof;
+  std::vector<double> host_partial;
+
+  ~ccq_hip_ctx() {
+    if (tables) (void)hipFree(tables);
+    if (itab) (void)hipFree(itab);
+    if (dtab) (void)hipFree(dtab);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int64_t dimsym(const std::string& s) const {
+    if (s == "oa") return oa;
+    if (s == "ob") return ob;
+    if (s == "va") return va;
+    if (s == "vb") return vb;
+    throw Fail("unknown dimension symbol " + s);
+  }
+
+  void layout() {
+    // ccq's t_vec (src/utils/cc_utils.f90:479-559): occupied-first C order
+    struct B {
+      const char* name;
+      std::vector<int64_t> d;
+    };
+    std::vector<B> blocks = {
+        {"t1a", {oa, va}},
+        {"t1b", {ob, vb}},
+        {"t2aa", {oa, oa, va, va}},
+        {"t2ab", {oa, ob, va, vb}},
+        {"t2bb", {ob, ob, vb, vb}},
+        {"t3aaa", {oa, oa, oa, va, va, va}},
+        {"t3aab", {oa, oa, ob, va, va, vb}},
+        {"t3abb", {oa, ob, ob, va, vb, vb}},
+        {"t3bbb", {ob, ob, ob, vb, vb, vb}},
+    };
+    int64_t off = 0;
+    for (auto& b : blocks) {
+      int64_t sz = 1;
+      for (auto x : b.d) sz *= x;
+      blk_off[b.name] = off;
+      blk_size[b.name] = sz;
+      blk_dims[b.name] = b.d;
+      off += sz;
+    }
+    t_size = off;
+    vec_size = (level == 2) ? blk_off["t3aaa"] : t_size;
+    dev_t = (level == 2) ? blk_off["t3aaa"] : t_size;
+  }
+
+  // -------------------------------------------------------------------------
+  // IR loading
+
+  void load_ir(const char* text) {
+    std::istringstream in(text);
+    std::string line;
+    Section* cur = nullptr;
+    while (std::getline(in, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::istringstream ls(line);
+      std::string kw;
+      ls >> kw;
+      if (kw == "ccq-ir" || kw == "end") continue;
+      if (kw == "tensor") {
+        Tensor t;
+        std::string kind;
+        ls >> t.id >> kind;
+        if (kind == "inter") {
+          t.kind = TKind::Inter;
+          ls >> t.rank;
+          for (int i = 0; i < t.rank; ++i) {
+            std::string s;
+            ls >> s;
+            t.dims[i] = dimsym(s);
+          }
+        } else if (kind == "view") {
+          t.kind = TKind::View;
+          ls >> t.base >> t.rank;
+          for (int i = 0; i < t.rank; ++i) ls >> t.spec[i];
+        } else if (kind == "amp" || kind == "resid") {
+          t.kind = kind == "amp" ? TKind::Amp : TKind::Resid;
+          ls >> t.base;
+          auto& d = blk_dims.at(t.base);
+          t.rank = (int)d.size();
+          for (int i = 0; i < t.rank; ++i) t.dims[i] = d[i];
+        } else {
+          throw Fail("bad tensor kind " + kind);
+        }
+        tindex[t.id] = (int)tensors.size();
+        tensors.push_back(t);
+      } else if (kw == "section") {
+        sections.emplace_back();
+        cur = &sections.back();
+        ls >> cur->name;
+      } else if (kw == "op") {
+        Op op;
+        std::string out, a, b;
+        int assign, nin;
+        ls >> out >> assign >> op.coef >> nin;
+        op.out = tindex.at(out);
+        op.assign = assign != 0;
+        ls >> a >> op.la;
+        op.a = tindex.at(a);
+        if (nin == 2) {
+          ls >> b >> op.lb;
+          op.b = tindex.at(b);
+          op.gemm = true;
+        }
+        ls >> op.lc;
+        cur->kind = Section::Compute;
+        cur->ops.push_back(op);
+      } else if (kw == "update") {
+        cur->kind = Section::Update;
+        std::string rid;
+        ls >> cur->upd.block >> rid >> cur->upd.nimg;
+        cur->upd.resid = tindex.at(rid);
+        for (int g = 0; g < cur->upd.nimg; ++g) {
+          std::string tok;
+          ls >> tok;
+          auto colon = tok.find(':');
+          std::string ps = tok.substr(0, colon);
+          double sg = std::stod(tok.substr(colon + 1));
+          std::vector<int> p(8, 0);
+          std::istringstream pss(ps);
+          std::string x;
+          int k = 0;
+          while (std::getline(pss, x, ',')) p[k++] = std::stoi(x);
+          cur->upd.perms.insert(cur->upd.perms.end(), p.begin(), p.end());
+          cur->upd.signs.push_back(sg);
+        }
+      } else if (kw == "copy") {
+        cur->kind = Section::Copy;
+        ls >> cur->dst >> cur->src;
+      } else {
+        throw Fail("bad IR keyword " + kw);
+      }
+    }
+  }
+
+  // -------------------------------------------------------------------------
+  // planning
+
+  void resolve_tensors() {
+    for (auto& t : tensors) {
+      if (t.kind == TKind::Inter) {
+        int64_t s = 1;
+        for (int i = t.rank - 1; i >= 0; --i) {
+          t.strides[i] = s;
+          s *= t.dims[i];
+        }
+        inter_buf.emplace_back(new DevBuf);
+        inter_buf.back()->alloc(std::max<int64_t>(s, 1));
+        t.ptr = inter_buf.back()->p;
+      } else if (t.kind == TKind::Amp || t.kind == TKind::Resid) {
+        int64_t s = 1;
+        for (int i = t.rank - 1; i >= 0; --i) {
+          t.strides[i] = s;
+          s *= t.dims[i];
+        }
+        if (t.kind == TKind::Amp) {
+          if (blk_off.at(t.base) + s > dev_t) throw Fail("amplitude block beyond device t_vec");
+          t.ptr = tvec.p + blk_off.at(t.base);
+        } else {
+          auto& rb = resid_buf[t.base];
+          if (!rb) {
+            rb.reset(new DevBuf);
+            rb->alloc(s);
+          }
+          t.ptr = rb->p;
+        }
+      } else {
+        // view into a full-range base array
+        double* basep = nullptr;
+        int64_t nb = 0;
+        if (t.base == "Vaa") basep = Vaa.p, nb = 4;
+        else if (t.base == "Vab") basep = Vab.p, nb = 4;
+        else if (t.base == "Vbb") basep = vbb_ptr, nb = 4;
+        else if (t.base == "Fa") basep = Fa.p, nb = 2;
+        else if (t.base == "Fb") basep = Fb.p, nb = 2;
+        else throw Fail("unknown base " + t.base);
+        if (nb != t.rank) throw Fail("view rank mismatch for " + t.id);
+        int64_t s = 1;
+        int64_t off = 0;
+        for (int i = t.rank - 1; i >= 0; --i) {
+          t.strides[i] = s;
+          const char sp = t.spec[i][0];
+          const bool beta = t.spec[i][1] == 'b';
+          const int64_t occ = beta ? ob : oa;
+          t.dims[i] = sp == 'o' ? occ : nc - occ;
+          if (sp == 'v') off += occ * s;
+          s *= nc;
+        }
+        t.ptr = basep + off;
+      }
+    }
+  }
+
+  static std::vector<int64_t> offsets(const Tensor& t, const std::string& labels,
+                                      const std::string& subset) {
+    std::vector<int64_t> ext, str;
+    for (char c : subset) {
+      auto pos = labels.find(c);
+      if (pos == std::string::npos) throw Fail("label not in tensor");
+      ext.push_back(t.dims[pos]);
+      str.push_back(t.strides[pos]);
+    }
+    int64_t total = 1;
+    for (auto e : ext) total *= e;
+    std::vector<int64_t> out((size_t)std::max<int64_t>(total, 1), 0);
+    if (subset.empty()) return out;
+    std::vector<int64_t> idx(ext.size(), 0);
+    for (int64_t n = 0; n < total; ++n) {
+      int64_t o = 0;
+      for (size_t k = 0; k < ext.size(); ++k) o += idx[k] * str[k];
+      out[n] = o;
+      for (int k = (int)ext.size() - 1; k >= 0; --k) {
+        if (++idx[k] < ext[k]) break;
+        idx[k] = 0;
+      }
+    }
+    return out;
+  }
+
+  static int64_t stride_of(const Tensor& t, const std::string& labels, char c) {
+    auto pos = labels.find(c);
+    return t.strides[pos];
+  }
+
+  void plan() {
+    std::vector<int64_t> host;
+    std::vector<int> ih;
+    std::vector<double> dh;
+    auto push = [&](const std::vector<int64_t>& v) {
+      int64_t o = (int64_t)host.size();
+      host.insert(host.end(), v.begin(), v.end());
+      return o;
+    };
+    for (auto& sec : sections) {
+      if (sec.kind == Section::Update) {
+        sec.upd.dev_perm_off = (int)ih.size();
+        ih.insert(ih.end(), sec.upd.perms.begin(), sec.upd.perms.end());
+        sec.upd.dev_sign_off = (int)dh.size();
+        dh.insert(dh.end(), sec.upd.signs.begin(), sec.upd.signs.end());
+        continue;
+      }
+      for (auto& op : sec.ops) {
+        const Tensor& C = tensors[op.out];
+        const Tensor& A = tensors[op.a];
+        for (size_t i = 0; i < op.lc.size(); ++i) {
+          // extents must agree label by label
+          char c = op.lc[i];
+          auto pa = op.la.find(c);
+          if (pa != std::string::npos && A.dims[pa] != C.dims[i]) throw Fail("extent mismatch A/C");
+          if (op.gemm) {
+            auto pb = op.lb.find(c);
+            if (pb != std::string::npos && tensors[op.b].dims[pb] != C.dims[i])
+              throw Fail("extent mismatch B/C");
+          }
+        }
+        std::string ms, ns, ks;
+        if (op.gemm) {
+          const Tensor& B = tensors[op.b];
+          for (char c : op.la) {
+            bool inC = op.lc.find(c) != std::string::npos;
+            bool inB = op.lb.find(c) != std::string::npos;
+            if (inC && inB) throw Fail("batch label unsupported in " + sec.name);
+            if (!inC && !inB) throw Fail("partial trace unsupported in " + sec.name);
+            if (!inC) ks += c;
+          }
+          for (char c : op.lb) {
+            bool inC = op.lc.find(c) != std::string::npos;
+            bool inA = op.la.find(c) != std::string::npos;
+            if (!inC && !inA) throw Fail("partial trace unsupported in " + sec.name);
+          }
+          for (char c : op.lc) {
+            if (op.la.find(c) != std::string::npos) ms += c;
+            else ns += c;
+          }
+          for (char c : ks) {
+            auto pa = op.la.find(c), pb = op.lb.find(c);
+            if (A.dims[pa] != B.dims[pb]) throw Fail("extent mismatch on summed label");
+          }
+          auto rA = offsets(A, op.la, ms), kA = offsets(A, op.la, ks);
+          auto kB = offsets(B, op.lb, ks), cB = offsets(B, op.lb, ns);
+          auto rC = offsets(C, op.lc, ms), cC = offsets(C, op.lc, ns);
+          op.M = (int64_t)rA.size();
+          op.N = (int64_t)cB.size();
+          op.K = (int64_t)kA.size();
+          op.t_rowA = push(rA);
+          op.t_kA = push(kA);
+          op.t_kB = push(kB);
+          op.t_colB = push(cB);
+          op.t_rowC = push(rC);
+          op.t_colC = push(cC);
+          op.a_kfast = ks.empty() ? 1 : (stride_of(A, op.la, ks.back()) == 1 ? 1
+                      : (!ms.empty() && stride_of(A, op.la, ms.back()) == 1 ? 0 : 1));
+          op.b_kfast = ks.empty() ? 1 : (stride_of(B, op.lb, ks.back()) == 1 ? 1
+                      : (!ns.empty() && stride_of(B, op.lb, ns.back()) == 1 ? 0 : 1));
+          op.flops = 2.0 * (double)op.M * (double)op.N * (double)(ks.empty() ? 1 : op.K);
+          op.bytes = 8.0 * ((double)op.M * op.K + (double)op.K * op.N + 2.0 * op.M * op.N);
+          op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +
+                   std::to_string(op.K);
+        } else {
+          if (op.la.size() != op.lc.size()) throw Fail("add op must permute its input");
+          ms = op.lc.substr(0, op.lc.size() - 1);
+          ns = op.lc.substr(op.lc.size() - 1);
+          auto rA = offsets(A, op.la, ms), cA = offsets(A, op.la, ns);
+          auto rC = offsets(C, op.lc, ms), cC = offsets(C, op.lc, ns);
+          op.M = (int64_t)rA.size();
+          op.N = (int64_t)cA.size();
+          op.t_rowA = push(rA);
+          op.t_kA = push(cA);
+          op.t_rowC = push(rC);
+          op.t_colC = push(cC);
+          op.flops = (double)op.M * op.N;
+          op.bytes = 8.0 * 3.0 * op.M * op.N;
+          op.cls = "add";
+        }
+      }
+    }
+    HIPCHK(hipMalloc(&tables, sizeof(int64_t) * std::max<size_t>(host.size(), 1)));
+    if (!host.empty())
+      HIPCHK(hipMemcpy(tables, host.data(), sizeof(int64_t) * host.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&itab, sizeof(int) * std::max<size_t>(ih.size(), 1)));
+    if (!ih.empty()) HIPCHK(hipMemcpy(itab, ih.data(), sizeof(int) * ih.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&dtab, sizeof(double) * std::max<size_t>(dh.size(), 1)));
+    if (!dh.empty())
+      HIPCHK(hipMemcpy(dtab, dh.data(), sizeof(double) * dh.size(), hipMemcpyHostToDevice));
+  }
+
+  // -------------------------------------------------------------------------
+  // execution
+
+  void run_op(const Op& op) {
+    const Tensor& C = tensors[op.out];
+    const Tensor& A = tensors[op.a];
+    const int acc = op.assign ? 0 : 1;
+    if (op.gemm) {
+      const Tensor& B = tensors[op.b];
+      ccqk::GemmArgs g;
+      g.M = op.M;
+      g.N = op.N;
+      g.K = op.K;
+      g.A = A.ptr;
+      g.rowA = tables + op.t_rowA;
+      g.kA = tables + op.t_kA;
+      g.B = B.ptr;
+      g.kB = tables + op.t_kB;
+      g.colB = tables + op.t_colB;
+      g.C = C.ptr;
+      g.rowC = tables + op.t_rowC;
+      g.colC = tables + op.t_colC;
+      g.alpha = op.coef;
+      g.accumulate = acc;
+      g.a_kfast = op.a_kfast;
+      g.b_kfast = op.b_kfast;
+      HIPCHK(ccqk::launch_gemm(stream, g));
+    } else {
+      HIPCHK(ccqk::launch_perm_add(stream, op.M, op.N, A.ptr, tables + op.t_rowA, tables + op.t_kA,
+                                   C.ptr, tables + op.t_rowC, tables + op.t_colC, op.coef, acc));
+    }
+  }
+
+  void run_update(const UpdateStep& u) {
+    const Tensor& R = tensors[u.resid];
+    ccqk::UpdateParams P{};
+    P.rank = R.rank;
+    P.nimg = u.nimg;
+    P.total = 1;
+    const std::string& b = u.block;
+    // positions: first half occupied, second half virtual; spin pattern by block name
+    const int rank = R.rank, half = rank / 2;
+    std::string spins = b.substr(2);  // e.g. "aab" from t3aab, "a" from t1a
+    if (spins.size() == 1 && rank == 2) spins = spins;  // t1a -> "a"
+    for (int d = 0; d < rank; ++d) {
+      P.dims[d] = R.dims[d];
+      P.strides[d] = R.strides[d];
+      P.total *= R.dims[d];
+      const int k = d % half;
+      const bool beta = spins[k] == 'b';
+      P.spin[d] = beta ? 1 : 0;
+      P.isvir[d] = d >= half ? 1 : 0;
+      P.occ[d] = beta ? ob : oa;
+      P.group[d] = -1;
+    }
+    // antisymmetric groups: same-spin runs within the occupied / virtual halves
+    for (int h = 0; h < 2; ++h)
+      for (int k = 0; k + 1 < half; ++k)
+        if (spins[k] == spins[k + 1]) {
+          P.group[h * half + k] = h * 16 + (spins[k] == 'b');
+          P.group[h * half + k + 1] = h * 16 + (spins[k] == 'b');
+        }
+    P.shift = shift;
+    double* t = tvec.p + blk_off.at(b);
+    HIPCHK(ccqk::launch_update(stream, P, t, R.ptr, itab + u.dev_perm_off, dtab + u.dev_sign_off,
+                               fdiag_a.p, fdiag_b.p));
+  }
+
+  void run_copy(const std::string& dst, const std::string& src) {
+    if (blk_size.at(dst) != blk_size.at(src)) throw Fail("copy between blocks of different size");
+    HIPCHK(hipMemcpyAsync(tvec.p + blk_off.at(dst), tvec.p + blk_off.at(src),
+                          sizeof(double) * blk_size.at(src), hipMemcpyDeviceToDevice, stream));
+  }
+
+  void sweep(int64_t nsec = -1) {
+    if (!ints_ready) throw Fail("integrals not uploaded");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (profile) {
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+    }
+    auto timed = [&](const std::string& cls, double fl, double by, auto&& fn) {
+      if (!profile) {
+        fn();
+        return;
+      }
+      HIPCHK(hipEventRecord(e0, stream));
+      fn();
+      HIPCHK(hipEventRecord(e1, stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      auto& p = prof[cls];
+      p.calls++;
+      p.ms += ms;
+      p.flops += fl;
+      p.bytes += by;
+    };
+    int64_t isec = 0;
+    for (auto& sec : sections) {
+      if (nsec >= 0 && isec++ >= nsec) break;
+      if (sec.kind == Section::Compute) {
+        for (auto& op : sec.ops) timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
+      } else if (sec.kind == Section::Update) {
+        const Tensor& R = tensors[sec.upd.resid];
+        double n = 1;
+        for (int d = 0; d < R.rank; ++d) n *= R.dims[d];
+        timed("update:" + sec.upd.block, 0, 8.0 * 3 * n, [&] { run_update(sec.upd); });
+      } else {
+        run_copy(sec.dst, sec.src);
+      }
+    }
+    if (profile) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
+  }
+
+  double reduce_partials(int count) {
+    host_partial.resize(count);
+    HIPCHK(hipMemcpyAsync(host_partial.data(), partial.p, sizeof(double) * count,
+                          hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    double s = 0.0;
+    for (int i = 0; i < count; ++i) s += host_partial[i];
+    return s;
+  }
+
+  double energy() {
+    // src/energy.f90:138-270, same eight pieces
+    double* t1a = tvec.p + blk_off["t1a"];
+    double* t1b = tvec.p + blk_off["t1b"];
+    double total = 0.0;
+    HIPCHK(ccqk::launch_dot_strided(stream, oa, va, Fa.p + oa, nc, t1a, partial.p, kPartialBlocks));
+    total += reduce_partials(kPartialBlocks);
+    HIPCHK(ccqk::launch_dot_strided(stream, ob, vb, Fb.p + ob, nc, t1b, partial.p, kPartialBlocks));
+    total += reduce_partials(kPartialBlocks);
+    struct Case {
+      const double* V;
+      int64_t o1, o2;
+      const char* t2;
+      double* x;
+      double* y;
+      double w2, w11;
+    };
+    Case cases[3] = {{Vaa.p, oa, oa, "t2aa", t1a, t1a, 0.25, 0.5},
+                     {Vab.p, oa, ob, "t2ab", t1a, t1b, 1.0, 1.0},
+                     {vbb_ptr, ob, ob, "t2bb", t1b, t1b, 0.25, 0.5}};
+    for (auto& c : cases) {
+      ccqk::EnergyParams P{};
+      P.V = c.V;
+      P.vs0 = nc * nc * nc;
+      P.vs1 = nc * nc;
+      P.vs2 = nc;
+      P.vs3 = 1;
+      P.vo1 = c.o1;
+      P.vo2 = c.o2;
+      P.t2 = tvec.p + blk_off[c.t2];
+      P.t1x = c.x;
+      P.t1y = c.y;
+      P.o1 = c.o1;
+      P.o2 = c.o2;
+      P.v1 = nc - c.o1;
+      P.v2 = nc - c.o2;
+      P.w2 = c.w2;
+      P.w11 = c.w11;
+      HIPCHK(ccqk::launch_energy_pairs(stream, P, partial.p, kPartialBlocks));
+      total += reduce_partials(kPartialBlocks);
+    }
+    return total;
+  }
+
+  int64_t slot_of(int64_t iter) const {
+    int64_t s = iter % (diis + 1);
+    if (s == 0) s = diis + 1;
+    return s - 1;  // zero-based slot
+  }
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+
+namespace {
+
+// LU with partial pivoting (the dgesv algorithm) for the small DIIS system.
+bool solve_dense(std::vector<double> A, std::vector<double>& b, int n) {
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    double best = std::fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(A[i * n + k]) > best) best = std::fabs(A[i * n + k]), piv = i;
+    if (best == 0.0) return false;
+    if (piv != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[piv * n + j]);
+      std::swap(b[k], b[piv]);
+    }
+    for (int i = k + 1; i < n; ++i) {
+      const double f = A[i * n + k] / A[k * n + k];
+      A[i * n + k] = f;
+      for (int j = k + 1; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+      b[i] -= f * b[k];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s -= A[i * n + j] * b[j];
+    b[i] = s / A[i * n + i];
+  }
+  return true;
+}
+
+template <class F>
+int guarded(ccq_hip_ctx* ctx, F&& fn) {
+  if (!ctx) return 2;
+  try {
+    fn();
+    return 0;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return 1;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_ctx** out) {
+  if (!dims || !opts || !out) return 2;
+  *out = nullptr;
+  std::unique_ptr<ccq_hip_ctx> ctx(new ccq_hip_ctx);
+  int rc = guarded(ctx.get(), [&] {
+    ctx->froz = dims->froz;
+    ctx->occ_a = dims->occ_a;
+    ctx->occ_b = dims->occ_b;
+    ctx->orbs = dims->orbs;
+    ctx->oa = dims->occ_a - dims->froz;
+    ctx->ob = dims->occ_b - dims->froz;
+    ctx->nc = dims->orbs - dims->froz;
+    ctx->va = dims->orbs - dims->occ_a;
+    ctx->vb = dims->orbs - dims->occ_b;
+    if (ctx->oa < 0 || ctx->ob < 0 || ctx->va < 0 || ctx->vb < 0 || ctx->nc <= 0)
+      throw Fail("inconsistent orbital counts");
+    ctx->level = (int)opts->level;
+    if (ctx->level != 2) throw Fail("only CCSD (level 2) is built into this library");
+    ctx->rhf = opts->rhf != 0;
+    if (ctx->rhf && ctx->oa != ctx->ob) throw Fail("rhf requires occ_a == occ_b");
+    ctx->shift = opts->shift;
+    ctx->diis = (int)opts->diis_space;
+    if (ctx->diis < 1 || ctx->diis > ccqk::kMaxDiis) throw Fail("diis_space must be in 1..12");
+    ctx->device = (int)opts->device;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->layout();
+    const int64_t nc4 = ctx->nc * ctx->nc * ctx->nc * ctx->nc;
+    ctx->Vaa.alloc(nc4);
+    ctx->Vab.alloc(nc4);
+    ctx->Fa.alloc(ctx->nc * ctx->nc);
+    ctx->Fb.alloc(ctx->nc * ctx->nc);
+    ctx->fdiag_a.alloc(ctx->nc);
+    ctx->fdiag_b.alloc(ctx->nc);
+    ctx->tvec.alloc(ctx->dev_t);
+    HIPCHK(hipMemsetAsync(ctx->tvec.p, 0, sizeof(double) * ctx->dev_t, ctx->stream));
+    ctx->ring.alloc((int64_t)(ctx->diis + 1) * ctx->vec_size);
+    ctx->partial.alloc(kPartialBlocks * 128);
+    ctx->load_ir(ctx->rhf ? kIR_CCSD_RHF : kIR_CCSD_UHF);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+  if (rc != 0) {
+    fprintf(stderr, "ccq_hip_create: %s\n", ctx->err.c_str());
+    return rc;
+  }
+  *out = ctx.release();
+  return 0;
+}
+
+int ccq_hip_set_ints(ccq_hip_ctx* ctx, const double* f_a, const double* f_b, const double* v_aa,
+                     const double* v_ab, const double* v_bb) {
+  return guarded(ctx, [&] {
+    const int64_t nc = ctx->nc, n3 = ctx->orbs, n0 = ctx->froz;
+    const int64_t nc4 = nc * nc * nc * nc;
+    // Fortran v(p,q,r,s) column-major -> C order [p][q][r][s]: full axis reversal
+    std::vector<double> buf((size_t)nc4);
+    auto rev4 = [&](const double* src, DevBuf& dst) {
+      for (int64_t p = 0; p < nc; ++p)
+        for (int64_t q = 0; q < nc; ++q)
+          for (int64_t r = 0; r < nc; ++r) {
+            double* o = buf.data() + ((p * nc + q) * nc + r) * nc;
+            const double* s = src + p + nc * q + nc * nc * r;
+            for (int64_t x = 0; x < nc; ++x) o[x] = s[x * nc * nc * nc];
+          }
+      HIPCHK(hipMemcpy(dst.p, buf.data(), sizeof(double) * nc4, hipMemcpyHostToDevice));
+    };
+    rev4(v_aa, ctx->Vaa);
+    rev4(v_ab, ctx->Vab);
+    // ccq builds v_bb identically to v_aa (src/integrals.f90:286-287); keep one
+    // copy when they agree.
+    const bool same = v_bb == v_aa || std::memcmp(v_bb, v_aa, sizeof(double) * nc4) == 0;
+    if (!ctx->ints_ready) {
+      if (same) {
+        ctx->vbb_ptr = ctx->Vaa.p;
+      } else {
+        ctx->Vbb.alloc(nc4);
+        ctx->vbb_ptr = ctx->Vbb.p;
+      }
+    } else if (same != (ctx->vbb_ptr == ctx->Vaa.p)) {
+      throw Fail("v_bb changed between uploads");
+    }
+    if (!same) rev4(v_bb, ctx->Vbb);
+    std::vector<double> fa(nc * nc), fb(nc * nc), da(nc), db(nc);
+    for (int64_t p = 0; p < nc; ++p)
+      for (int64_t q = 0; q < nc; ++q) {
+        fa[p * nc + q] = f_a[(n0 + p) + n3 * (n0 + q)];
+        fb[p * nc + q] = f_b[(n0 + p) + n3 * (n0 + q)];
+      }
+    for (int64_t p = 0; p < nc; ++p) da[p] = fa[p * nc + p], db[p] = fb[p * nc + p];
+    HIPCHK(hipMemcpy(ctx->Fa.p, fa.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->Fb.p, fb.data(), sizeof(double) * nc * nc, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->fdiag_a.p, da.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->fdiag_b.p, db.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+    if (!ctx->ints_ready) {
+      ctx->resolve_tensors();
+      ctx->plan();
+    }
+    ctx->ints_ready = true;
+  });
+}
+
+int64_t ccq_hip_t_size(const ccq_hip_ctx* ctx) { return ctx ? ctx->t_size : -1; }
+
+int ccq_hip_set_t(ccq_hip_ctx* ctx, const double* t_vec, int64_t t_size) {
+  return guarded(ctx, [&] {
+    if (t_size != ctx->t_size) throw Fail("t_size mismatch");
+    HIPCHK(hipMemcpyAsync(ctx->tvec.p, t_vec, sizeof(double) * ctx->dev_t, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ccq_hip_get_t(ccq_hip_ctx* ctx, double* t_vec, int64_t t_size) {
+  return guarded(ctx, [&] {
+    if (t_size != ctx->t_size) throw Fail("t_size mismatch");
+    HIPCHK(hipMemcpyAsync(t_vec, ctx->tvec.p, sizeof(double) * ctx->dev_t, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // blocks not resident (T3 for CCSD) are zero, as in ccq
+    for (int64_t i = ctx->dev_t; i < ctx->t_size; ++i) t_vec[i] = 0.0;
+  });
+}
+
+int ccq_hip_update(ccq_hip_ctx* ctx) {
+  return guarded(ctx, [&] { ctx->sweep(); });
+}
+
+int ccq_hip_energy(ccq_hip_ctx* ctx, double* e) {
+  return guarded(ctx, [&] { *e = ctx->energy(); });
+}
+
+int ccq_hip_diis_record(ccq_hip_ctx* ctx, int64_t iter) {
+  return guarded(ctx, [&] {
+    const int64_t s = ctx->slot_of(iter);
+    HIPCHK(hipMemcpyAsync(ctx->ring.p + s * ctx->vec_size, ctx->tvec.p,
+                          sizeof(double) * ctx->vec_size, hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}
+
+int ccq_hip_residuum(ccq_hip_ctx* ctx, int64_t iter, double* res) {
+  return guarded(ctx, [&] {
+    const double* cur = ctx->ring.p + ctx->slot_of(iter) * ctx->vec_size;
+    const double* prev = iter == 1 ? nullptr : ctx->ring.p + ctx->slot_of(iter - 1) * ctx->vec_size;
+    HIPCHK(ccqk::launch_diff_sq(ctx->stream, prev, cur, ctx->vec_size, ctx->partial.p,
+                                kPartialBlocks));
+    *res = std::sqrt(std::sqrt(ctx->reduce_partials(kPartialBlocks)));
+  });
+}
+
+int ccq_hip_diis_extrapolate(ccq_hip_ctx* ctx) {
+  return guarded(ctx, [&] {
+    const int n = ctx->diis;
+    const int np = n * (n + 1) / 2;
+    HIPCHK(ccqk::launch_diis_gram(ctx->stream, ctx->ring.p, ctx->vec_size, n, ctx->partial.p,
+                                  kPartialBlocks));
+    ctx->host_partial.resize((size_t)kPartialBlocks * np);
+    HIPCHK(hipMemcpyAsync(ctx->host_partial.data(), ctx->partial.p,
+                          sizeof(double) * kPartialBlocks * np, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<double> g(np, 0.0);
+    for (int b = 0; b < kPartialBlocks; ++b)
+      for (int q = 0; q < np; ++q) g[q] += ctx->host_partial[(size_t)b * np + q];
+    const int m = n + 1;
+    std::vector<double> B((size_t)m * m, 0.0), c(m, 0.0);
+    int q = 0;
+    for (int x = 0; x < n; ++x)
+      for (int y = x; y < n; ++y) {
+        B[x * m + y] = g[q];
+        B[y * m + x] = g[q];
+        ++q;
+      }
+    for (int i = 0; i < m; ++i) {
+      B[i * m + n] = -1.0;
+      B[n * m + i] = -1.0;
+    }
+    c[n] = -1.0;
+    if (!solve_dense(B, c, m)) throw Fail("calc_diis: DIIS error in DGESV.");
+    ccqk::Coeffs cf{};
+    for (int k = 0; k < n; ++k) cf.c[k] = c[k];
+    HIPCHK(ccqk::launch_lincomb(ctx->stream, ctx->tvec.p, ctx->ring.p, ctx->vec_size, n, cf));
+  });
+}
+
+int ccq_hip_profile(ccq_hip_ctx* ctx, int64_t enable) {
+  return guarded(ctx, [&] {
+    ctx->profile = enable != 0;
+    ctx->prof.clear();
+  });
+}
+
+int ccq_hip_profile_report(ccq_hip_ctx* ctx, char* buf, int64_t buflen) {
+  return guarded(ctx, [&] {
+    std::ostringstream os;
+    for (auto& kv : ctx->prof)
+      os << kv.first << " " << kv.second.calls << " " << kv.second.ms << " " << kv.second.flops
+         << " " << kv.second.bytes << "\n";
+    std::string s = os.str();
+    if ((int64_t)s.size() + 1 > buflen) throw Fail("profile buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+  });
+}
+
+int ccq_hip_sync(ccq_hip_ctx* ctx) {
+  return guarded(ctx, [&] { HIPCHK(hipStreamSynchronize(ctx->stream)); });
+}
+
+int ccq_hip_debug_sweep(ccq_hip_ctx* ctx, int64_t nsections) {
+  return guarded(ctx, [&] { ctx->sweep(nsections); });
+}
+
+int ccq_hip_debug_read(ccq_hip_ctx* ctx, const char* tensor_id, double* out, int64_t n) {
+  return guarded(ctx, [&] {
+    auto it = ctx->tindex.find(tensor_id);
+    if (it == ctx->tindex.end()) throw Fail(std::string("no tensor ") + tensor_id);
+    const Tensor& t = ctx->tensors[it->second];
+    if (t.kind == TKind::View) throw Fail("views are not readable");
+    int64_t total = 1;
+    for (int d = 0; d < t.rank; ++d) total *= t.dims[d];
+    if (n != total) throw Fail("debug_read size mismatch");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(out, t.ptr, sizeof(double) * total, hipMemcpyDeviceToHost));
+  });
+}
+
+const char* ccq_hip_last_error(const ccq_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ccq_hip_destroy(ccq_hip_ctx* ctx) {
+  if (!ctx) return 0;
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return 0;
+}
+
+}  // extern "C"

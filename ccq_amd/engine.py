@@ -1,0 +1,202 @@
+"""ctypes binding of the HIP amplitude engine (include/ccq_hip.h).
+
+This mirrors ccq's plug-in operator interface for the CC iteration:
+``update`` is update_ptr (src/proc_pointers.f90:8-32 -> update_clusters_t4),
+``energy`` is calculate_energy_ptr (calculate_unsorted_energy), and the DIIS
+calls are write_vecs / residuum / calc_diis (src/diis.F90).  Errors raise
+``EngineError`` carrying the library's message, like ccq's stop_all.
+
+The library must have been built (``python ccq_amd/build.py`` or
+``__graft_entry__.build()``); there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libccq_hip.so")
+
+LEVELS = {"CCSD": 2, "CCSDT": 3, "CCSDTQ": 4}
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("froz", ctypes.c_int64), ("occ_a", ctypes.c_int64),
+                ("occ_b", ctypes.c_int64), ("orbs", ctypes.c_int64)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_int64), ("rhf", ctypes.c_int64), ("shift", ctypes.c_double),
+                ("diis_space", ctypes.c_int64), ("device", ctypes.c_int64)]
+
+
+_lib = None
+
+EXPORTS = {
+    "ccq_hip_create": (ctypes.c_int, [ctypes.POINTER(Dims), ctypes.POINTER(Opts), ctypes.POINTER(ctypes.c_void_p)]),
+    "ccq_hip_set_ints": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5),
+    "ccq_hip_t_size": (ctypes.c_int64, [ctypes.c_void_p]),
+    "ccq_hip_set_t": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_get_t": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_update": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccq_hip_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_diis_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_residuum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_diis_extrapolate": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccq_hip_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_profile_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
+    "ccq_hip_debug_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_debug_sweep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccq_hip_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "ccq_hip_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+}
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineError(f"{path} is missing: build it with `python ccq_amd/build.py`")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Engine:
+    """One resident CC problem on one GPU."""
+
+    def __init__(self, froz: int, occ_a: int, occ_b: int, orbs: int, calc_type: str = "CCSD",
+                 rhf: bool = False, shift: float = 0.0, diis_space: int = 5, device: int = 0):
+        self.lib = load_library()
+        self.dims = Dims(froz, occ_a, occ_b, orbs)
+        self.opts = Opts(LEVELS[calc_type], int(rhf), float(shift), int(diis_space), int(device))
+        h = ctypes.c_void_p()
+        rc = self.lib.ccq_hip_create(ctypes.byref(self.dims), ctypes.byref(self.opts), ctypes.byref(h))
+        if rc != 0 or not h.value:
+            raise EngineError("ccq_hip_create failed (see stderr)")
+        self.h = h
+        self.calc_type = calc_type
+        self.t_size = int(self.lib.ccq_hip_t_size(self.h))
+
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.ccq_hip_last_error(self.h)
+            raise EngineError(f"{what}: {msg.decode() if msg else 'error'}")
+
+    def set_ints(self, f_a, f_b, v_aa, v_ab, v_bb):
+        """Arrays are the logical ccq arrays f(p,q), v(p,q,r,s) (any numpy order)."""
+        bufs = [np.asfortranarray(x, dtype=np.float64) for x in (f_a, f_b, v_aa, v_ab, v_bb)]
+        if v_bb is v_aa:
+            bufs[4] = bufs[2]
+        self._check(self.lib.ccq_hip_set_ints(self.h, *[_ptr(b) for b in bufs]), "set_ints")
+
+    def set_t(self, t_vec: np.ndarray):
+        t = np.ascontiguousarray(t_vec, dtype=np.float64)
+        self._check(self.lib.ccq_hip_set_t(self.h, _ptr(t), t.size), "set_t")
+
+    def get_t(self) -> np.ndarray:
+        out = np.empty(self.t_size)
+        self._check(self.lib.ccq_hip_get_t(self.h, _ptr(out), out.size), "get_t")
+        return out
+
+    def update(self):
+        self._check(self.lib.ccq_hip_update(self.h), "update")
+
+    def energy(self) -> float:
+        e = ctypes.c_double()
+        self._check(self.lib.ccq_hip_energy(self.h, ctypes.byref(e)), "energy")
+        return e.value
+
+    def diis_record(self, it: int):
+        self._check(self.lib.ccq_hip_diis_record(self.h, it), "diis_record")
+
+    def residuum(self, it: int) -> float:
+        r = ctypes.c_double()
+        self._check(self.lib.ccq_hip_residuum(self.h, it, ctypes.byref(r)), "residuum")
+        return r.value
+
+    def diis_extrapolate(self):
+        self._check(self.lib.ccq_hip_diis_extrapolate(self.h), "diis_extrapolate")
+
+    def profile(self, enable: bool):
+        self._check(self.lib.ccq_hip_profile(self.h, int(enable)), "profile")
+
+    def profile_report(self) -> dict:
+        buf = ctypes.create_string_buffer(1 << 20)
+        self._check(self.lib.ccq_hip_profile_report(self.h, buf, len(buf)), "profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, calls, ms, flops, nbytes = line.split()
+            out[name] = dict(calls=int(calls), ms=float(ms), flops=float(flops), bytes=float(nbytes))
+        return out
+
+    def debug_read(self, tensor_id: str, shape) -> np.ndarray:
+        out = np.empty(shape)
+        self._check(self.lib.ccq_hip_debug_read(self.h, tensor_id.encode(), _ptr(out), out.size),
+                    "debug_read")
+        return out
+
+    def debug_sweep(self, nsections: int):
+        self._check(self.lib.ccq_hip_debug_sweep(self.h, nsections), "debug_sweep")
+
+    def sync(self):
+        self._check(self.lib.ccq_hip_sync(self.h), "sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ccq_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# t_vec block layout (ccq's cc%pos, src/utils/cc_utils.f90:479-559)
+TVEC_BLOCKS = ['t1a', 't1b', 't2aa', 't2ab', 't2bb', 't3aaa', 't3aab', 't3abb', 't3bbb']
+
+
+def block_shapes(oa, ob, va, vb):
+    return {
+        't1a': (oa, va), 't1b': (ob, vb),
+        't2aa': (oa, oa, va, va), 't2ab': (oa, ob, va, vb), 't2bb': (ob, ob, vb, vb),
+        't3aaa': (oa, oa, oa, va, va, va), 't3aab': (oa, oa, ob, va, va, vb),
+        't3abb': (oa, ob, ob, va, vb, vb), 't3bbb': (ob, ob, ob, vb, vb, vb),
+    }
+
+
+def split_tvec(t_vec, oa, ob, va, vb) -> dict:
+    out = {}
+    off = 0
+    for name, shp in block_shapes(oa, ob, va, vb).items():
+        n = int(np.prod(shp))
+        out[name] = t_vec[off:off + n].reshape(shp)
+        off += n
+    return out
+
+
+def join_tvec(blocks: dict, oa, ob, va, vb) -> np.ndarray:
+    parts = []
+    for name, shp in block_shapes(oa, ob, va, vb).items():
+        b = blocks.get(name)
+        parts.append(np.zeros(int(np.prod(shp))) if b is None else np.ascontiguousarray(b).ravel())
+    return np.concatenate(parts)

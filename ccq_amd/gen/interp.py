@@ -1,0 +1,129 @@
+"""Numpy evaluator of compiled block programs (CPU debugging of the IR).
+
+Runs the same BlockOps the HIP executor runs, with ``numpy.einsum`` in place
+of the GEMM kernels, so a compiled program can be checked on the CPU before
+it is ever sent to a GPU.  Amplitude, integral and Fock arrays use the
+layouts of the C-ABI (C order; amplitudes occupied-first like ccq's t_vec).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .spinint import BlockKey, group_images
+
+AMP_BLOCKS = {
+    ('T1', (0, 0)): 't1a', ('T1', (1, 1)): 't1b',
+    ('T2', (0, 0, 0, 0)): 't2aa', ('T2', (0, 1, 0, 1)): 't2ab', ('T2', (1, 1, 1, 1)): 't2bb',
+    ('T3', (0, 0, 0, 0, 0, 0)): 't3aaa', ('T3', (0, 0, 1, 0, 0, 1)): 't3aab',
+    ('T3', (0, 1, 1, 0, 1, 1)): 't3abb', ('T3', (1, 1, 1, 1, 1, 1)): 't3bbb',
+}
+RES_OF = {'t1a': ('R1', (0, 0)), 't1b': ('R1', (1, 1)),
+          't2aa': ('R2', (0, 0, 0, 0)), 't2ab': ('R2', (0, 1, 0, 1)), 't2bb': ('R2', (1, 1, 1, 1))}
+
+
+class Dims:
+    def __init__(self, oa, ob, va, vb):
+        self.oa, self.ob, self.va, self.vb = oa, ob, va, vb
+        assert oa + va == ob + vb
+        self.n = oa + va
+
+    def occ(self, spin):
+        return self.oa if spin == 0 else self.ob
+
+    def extent(self, space, spin):
+        if space == 'o':
+            return self.occ(spin)
+        return self.n - self.occ(spin)
+
+
+def base_view(key: BlockKey, dims: Dims, base: dict):
+    """Slice of a full-range base array for a (spaces, spins) block."""
+    if key.name == 'F':
+        arr = base['Fa'] if key.spins[0] == 0 else base['Fb']
+    elif key.name == 'V':
+        sp = key.spins
+        if sp == (0, 0, 0, 0):
+            arr = base['Vaa']
+        elif sp == (1, 1, 1, 1):
+            arr = base['Vbb']
+        elif sp == (0, 1, 0, 1):
+            arr = base['Vab']
+        else:
+            raise KeyError(f"non-canonical V block {key}")
+    else:
+        raise KeyError(key)
+    idx = tuple(slice(0, dims.occ(s)) if c == 'o' else slice(dims.occ(s), None)
+                for c, s in zip(key.spaces, key.spins))
+    return arr[idx]
+
+
+def block_shape(key: BlockKey, dims: Dims):
+    return tuple(dims.extent(c, s) for c, s in zip(key.spaces, key.spins))
+
+
+class Interpreter:
+    def __init__(self, prog, sections, dims: Dims, base: dict, amps: dict,
+                 shift: float = 0.0):
+        self.prog, self.sections, self.dims = prog, sections, dims
+        self.base, self.amps = base, amps
+        self.shift = shift
+        self.store: dict[BlockKey, np.ndarray] = {}
+
+    def get(self, key: BlockKey):
+        kind = self.prog.decls[key.name].kind
+        if kind == 'base':
+            return base_view(key, self.dims, self.base)
+        if kind == 'amp':
+            return self.amps[AMP_BLOCKS[(key.name, key.spins)]]
+        return self.store[key]
+
+    def run_section(self, sec):
+        if sec.kind == 'compute':
+            for op in sec.ops:
+                out_shape = block_shape(op.out, self.dims)
+                terms = []
+                spec = ",".join(l for _, l in op.ins) + "->" + op.out_labels
+                val = np.einsum(spec, *[self.get(k) for k, _ in op.ins], optimize=True)
+                val = float(op.coef) * val
+                if op.assign or op.out not in self.store:
+                    self.store[op.out] = np.array(val, copy=True).reshape(out_shape)
+                else:
+                    self.store[op.out] += val
+                del terms
+        elif sec.kind == 'update':
+            self.update(sec.extra['block'], sec.target)
+        elif sec.kind == 'copy':
+            self.amps[sec.extra['dst']][...] = self.amps[sec.extra['src']]
+
+    def residual(self, block: str, key: BlockKey):
+        U = self.store[key]
+        decl = self.prog.decls[key.name]
+        imgs = group_images(decl, key)
+        R = np.zeros_like(U)
+        for perm, sign in imgs:
+            R += sign * U.transpose(perm)
+        return R / len(imgs)
+
+    def update(self, block: str, key: BlockKey):
+        R = self.residual(block, key)
+        D = self.denominators(key)
+        self.amps[block] -= R / D
+
+    def denominators(self, key: BlockKey):
+        fa = np.diag(self.base['Fa'])
+        fb = np.diag(self.base['Fb'])
+        D = np.zeros(block_shape(key, self.dims))
+        rank = len(key.spaces)
+        for p, (c, s) in enumerate(zip(key.spaces, key.spins)):
+            f = fa if s == 0 else fb
+            o = self.dims.occ(s)
+            e = f[:o] if c == 'o' else f[o:]
+            shp = [1] * rank
+            shp[p] = len(e)
+            D = D + (e if c == 'v' else -e).reshape(shp)
+        return D + self.shift
+
+    def sweep(self):
+        for sec in self.sections:
+            self.run_section(sec)

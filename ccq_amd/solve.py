@@ -1,0 +1,103 @@
+"""Host-side CC iteration over the HIP engine: ccq's jacobi_iter.
+
+Line-for-line the control flow of src/solve_cc.f90:175-269 -- update,
+energy, ring record, residuum, DIIS every diis_space+1 iterations, the
+iteration print, the ``cc.comm`` stop file (:271-291), the three-consecutive
+|dE| < tol convergence rule (:293-316), NaN exit and the max_iter message --
+with every amplitude operation running on the GPU through ``Engine``.
+Only scalars (energy, residuum) cross to the host per iteration.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+import sys
+import time
+
+from .engine import Engine
+
+
+@dataclasses.dataclass
+class SolveResult:
+    en_cor: float
+    iterations: int
+    converged: bool
+    failed: bool
+    history: list      # (iter, e_cor, dE, residuum, seconds)
+
+
+def fmt_iteration(it, e, de, res, secs):
+    """src/printing.F90:345 -- '(2x,i4,3(f15.10),i5,'' min'',f5.1,'' s'')'."""
+    nmin = int(secs) // 60
+    nsec = secs - nmin * 60.0
+    return f"  {it:4d}{e:15.10f}{de:15.10f}{res:15.10f}{nmin:5d} min{nsec:5.1f} s"
+
+
+def iter_header():
+    """src/printing.F90:303-310."""
+    return f"\n  {'It.':>4}{'E (Corr)':>15}{'dE':>15}{'Residuum':>15}{'Wall Time':>16}\n  " + "-" * 65
+
+
+def check_sig(path: str = "cc.comm") -> bool:
+    """src/solve_cc.f90:271-291: a file ``cc.comm`` stops the loop and is deleted."""
+    if os.path.exists(path):
+        os.remove(path)
+        return True
+    return False
+
+
+def jacobi_iter(eng: Engine, tol: float, max_iter: int, diis_space: int,
+                out=sys.stdout, on_iter=None, stop_file: str = "cc.comm") -> SolveResult:
+    conv = [1.0, 1.0, 1.0]
+    en_cor = 0.0
+    failed = False
+    converged = False
+    hist = []
+    prev = time.perf_counter()
+    it = 0
+    for it in range(1, max_iter + 1):
+        eng.update()
+        e_new = eng.energy()
+        de = en_cor - e_new
+        conv = [conv[1], conv[2], de]
+        en_cor = e_new
+        eng.diis_record(it)
+        res = eng.residuum(it)
+        if it % (diis_space + 1) == 0:
+            if out:
+                print("      DIIS cycle", file=out)
+            eng.diis_extrapolate()
+        now = time.perf_counter()
+        hist.append((it, e_new, de, res, now - prev))
+        if out:
+            print(fmt_iteration(it, e_new, de, res, now - prev), file=out, flush=True)
+        prev = now
+        if on_iter is not None:
+            on_iter(it, e_new)
+        if check_sig(stop_file):
+            if out:
+                print("Signaled to end", file=out)
+            break
+        if all(abs(c) < tol for c in conv):
+            converged = True
+            break
+        if math.isnan(e_new):
+            if out:
+                print("FAILED TO CONVERGE", file=out)
+            break
+        if it == max_iter:
+            if out:
+                print("FAILED TO CONVERGE IN", max_iter, " ITERATIONS. KEEPING LAST T VECTOR.", file=out)
+            failed = True
+    return SolveResult(en_cor=en_cor, iterations=it, converged=converged, failed=failed, history=hist)
+
+
+def summary(e_ref: float, e_cor: float, calc_type: str) -> str:
+    """src/printing.F90:260-284 -- '(a20,f18.12)' lines."""
+    lines = ["", "Calculation Summary (Eh)", "------------------------",
+             f"{'Reference':>20}{e_ref:18.12f}",
+             f"{'Correlation':>20}{e_cor:18.12f}",
+             f"{calc_type:>20}{e_ref + e_cor:18.12f}"]
+    return "\n".join(lines)

@@ -1,0 +1,109 @@
+/*
+ * ccq_hip.h -- C ABI of the MI355X (gfx950) coupled-cluster amplitude engine.
+ *
+ * Drop-in boundary for ccq's CC iteration (piecuch-group/ccq).  ccq calls its
+ * amplitude sweep and energy through the procedure pointers update_ptr /
+ * calculate_energy_ptr (src/proc_pointers.f90:8-32, bound in
+ * src/solve_cc.f90:47-54) from jacobi_iter (src/solve_cc.f90:175-269), which
+ * also writes the DIIS ring, prints the residuum and extrapolates
+ * (src/diis.F90).  Each entry point below replaces one of those calls; the
+ * integrals, T1..T4 and all intermediates stay resident on one GPU between
+ * calls and only scalars cross the boundary per iteration.
+ *
+ * Conventions (ccq is built with -fdefault-integer-8, config/gfortran.mk:10):
+ *   - every integer is int64_t, every real is double;
+ *   - arrays are Fortran column-major exactly as ccq holds them:
+ *       f_a, f_b      (1:orbs, 1:orbs)                    sys%ints%f_a/f_b
+ *       v_aa/ab/bb    (froz+1:orbs)^4, <pq|rs> ordering    sys%ints%v_* (src/integrals.f90:280-290)
+ *       t_vec         cc%t_vec, blocks T1A,T1B,T2A,T2B,T2C,T3A,T3B,T3C,T3D at
+ *                     cc%pos(1..9) (src/utils/cc_utils.f90:479-559)
+ *   - every call returns 0 on success and non-zero on error; the message is
+ *     available from ccq_hip_last_error (the Fortran side maps it to stop_all,
+ *     src/errors.f90:11-27).
+ * A context is not thread-safe; it owns one HIP stream on its device.
+ */
+#ifndef CCQ_HIP_H
+#define CCQ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ccq_hip_ctx ccq_hip_ctx;
+
+/* sys_t sizes (src/system.f90:60-106; src/parser.f90:476-504). */
+typedef struct {
+  int64_t froz;        /* n0: frozen spatial orbitals              */
+  int64_t occ_a;       /* n1: occupied alpha (incl. frozen)        */
+  int64_t occ_b;       /* n2: occupied beta  (incl. frozen)        */
+  int64_t orbs;        /* n3: spatial orbitals                     */
+} ccq_hip_dims;
+
+/* run_t options used by the iteration (src/system.f90:108-170). */
+typedef struct {
+  int64_t level;       /* 2 CCSD, 3 CCSDT, 4 CCSDTQ (run%lvl_t, run%lvl_q) */
+  int64_t rhf;         /* run%rhf: closed-shell copies T2C=T2A, T1B=T1A ... */
+  double shift;        /* run%shift: denominator shift                     */
+  int64_t diis_space;  /* run%diis_space (<= 12)                           */
+  int64_t device;      /* HIP device ordinal                               */
+} ccq_hip_opts;
+
+/* Create a context: allocate the resident tensors and plan the program.
+ * Replaces the set-up part of solve_cc (src/solve_cc.f90:12-99). */
+int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_ctx** out);
+
+/* Upload Fock and antisymmetrised integrals in ccq's layouts
+ * (sys%ints%f_a, f_b, v_aa, v_ab, v_bb after load_ints, src/integrals.f90:212-298). */
+int ccq_hip_set_ints(ccq_hip_ctx* ctx, const double* f_a, const double* f_b,
+                     const double* v_aa, const double* v_ab, const double* v_bb);
+
+/* Number of doubles in ccq's t_vec (cc%t_size = pos(10)-1). */
+int64_t ccq_hip_t_size(const ccq_hip_ctx* ctx);
+
+/* Copy cc%t_vec to / from the device (T1..T3; T4 stays resident). */
+int ccq_hip_set_t(ccq_hip_ctx* ctx, const double* t_vec, int64_t t_size);
+int ccq_hip_get_t(ccq_hip_ctx* ctx, double* t_vec, int64_t t_size);
+
+/* One in-place amplitude sweep in ccq's order: replaces update_ptr =>
+ * update_clusters_t4 (src/update_cc.F90:12-323). */
+int ccq_hip_update(ccq_hip_ctx* ctx);
+
+/* Correlation energy: replaces calculate_energy_ptr =>
+ * calculate_unsorted_energy (src/energy.f90:138-270). */
+int ccq_hip_energy(ccq_hip_ctx* ctx, double* e_cor);
+
+/* DIIS ring record of the current vector: write_vecs (src/diis.F90:289-349). */
+int ccq_hip_diis_record(ccq_hip_ctx* ctx, int64_t iter);
+
+/* Printed residuum sqrt(sqrt(|t_iter - t_iter-1|^2)): residuum (src/diis.F90:86-157). */
+int ccq_hip_residuum(ccq_hip_ctx* ctx, int64_t iter, double* res);
+
+/* DIIS extrapolation over the ring: calc_diis (src/diis.F90:11-84). */
+int ccq_hip_diis_extrapolate(ccq_hip_ctx* ctx);
+
+/* Optional per-operation timing with HIP events on the context's stream.
+ * ccq has no counterpart (it prints only wall time per iteration). */
+int ccq_hip_profile(ccq_hip_ctx* ctx, int64_t enable);
+/* Writes one line per op class: "<name> <calls> <total_ms> <flops> <bytes>". */
+int ccq_hip_profile_report(ccq_hip_ctx* ctx, char* buf, int64_t buflen);
+
+/* Copy one dense tensor of the loaded program (an intermediate, residual
+ * accumulator or amplitude block, by its IR id) to the host.  Inspection
+ * only; ccq has no counterpart. */
+int ccq_hip_debug_read(ccq_hip_ctx* ctx, const char* tensor_id, double* out, int64_t n);
+/* Run only the first nsections sections of one sweep (inspection only). */
+int ccq_hip_debug_sweep(ccq_hip_ctx* ctx, int64_t nsections);
+
+/* Block until the context's stream is idle. */
+int ccq_hip_sync(ccq_hip_ctx* ctx);
+
+const char* ccq_hip_last_error(const ccq_hip_ctx* ctx);
+int ccq_hip_destroy(ccq_hip_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCQ_HIP_H */

@@ -234,12 +234,27 @@ class Result:
     t: dict
 
 
+def block_denominators(t, fock_a, fock_b, oa, ob, shift):
+    fa_o, fa_v = np.diag(fock_a)[:oa], np.diag(fock_a)[oa:]
+    fb_o, fb_v = np.diag(fock_b)[:ob], np.diag(fock_b)[ob:]
+    return {name: denominators(name, fa_o, fa_v, fb_o, fb_v, shift) for name in t}
+
+
+def sweep(t, seq, residual, D):
+    """One in-place update_clusters_t4 sweep (src/update_cc.F90:102-318)."""
+    for item in seq:
+        if isinstance(item, tuple):
+            _, dst, fn, src = item
+            t[dst][...] = fn(t[src])
+        else:
+            R = residual(item, t)
+            t[item] -= R / D[item]
+
+
 def jacobi_iter(t, opts: Options, residual, fock_a, fock_b, Vaa, Vab, Vbb,
                 oa, ob, log=None) -> Result:
     """src/solve_cc.f90:175-269.  ``t`` is updated in place."""
-    fa_o, fa_v = np.diag(fock_a)[:oa], np.diag(fock_a)[oa:]
-    fb_o, fb_v = np.diag(fock_b)[:ob], np.diag(fock_b)[ob:]
-    D = {name: denominators(name, fa_o, fa_v, fb_o, fb_v, opts.shift) for name in t}
+    D = block_denominators(t, fock_a, fock_b, oa, ob, opts.shift)
     seq = sweep_order(opts.calc_type, opts.rhf)
     n = opts.diis_space
     ring = [None] * (n + 1)
@@ -249,13 +264,7 @@ def jacobi_iter(t, opts: Options, residual, fock_a, fock_b, Vaa, Vab, Vbb,
     converged = failed = False
     it = 0
     for it in range(1, opts.max_iter + 1):
-        for item in seq:
-            if isinstance(item, tuple):
-                _, dst, fn, src = item
-                t[dst][...] = fn(t[src])
-            else:
-                R = residual(item, t)
-                t[item] -= R / D[item]
+        sweep(t, seq, residual, D)
         e_new = energy(fock_a, fock_b, Vaa, Vab, Vbb, oa, ob, t)
         de = en_cor - e_new
         conv = [conv[1], conv[2], de]
