@@ -1,0 +1,213 @@
+"""Benchmark: time-to-|dE|<1e-7 Eh of the CC iteration on one MI355X.
+
+Workload (N=1): closed-shell CCSD, 100 spatial orbitals (10 occupied,
+90 virtual), synthetic integrals (pq|rs) = sum_L B B with L = 200 from seed 1,
+tol-driven ccq iteration (diis_space 5, shift 0), plain FP64.
+
+Protocol
+  1. build the system and upload it (untimed);
+  2. W warm-up iterations from T = 0, then a tight untimed solve to get the
+     converged FP64 energy E_conv (tol 1e-11);
+  3. reset T = 0 and time exactly K iterations (update, energy, DIIS ring,
+     residuum, DIIS every 6th), synchronising the stream around the timed
+     region and after every iteration (the energy is read back each time);
+  4. value = wall time until the first iteration with |E - E_conv| < 1e-7.
+  Roofline: the dominant kernel (the o^2 v^4 ladder GEMM, M=v^2 N=o^2 K=v^2)
+  timed with HIP events on the engine's stream during one extra profiled
+  iteration; algorithmic FLOPs 2*M*N*K per launch.
+  cpu_baseline: the numpy oracle (same equations, same sweep, OpenBLAS) on a
+  bounded sample of the same workload, extrapolated to the same iteration count.
+
+Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from ccq_amd import engine as E  # noqa: E402
+from ccq_amd import synth  # noqa: E402
+from ccq_amd.inputs import build_integrals  # noqa: E402
+
+METRIC = "time-to-|dE|<1e-7 Eh, mixed+learned vs plain FP64 (1 GPU); % FP64 MFMA peak"
+PEAK_FP64_MFMA = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(cfg_idx: int):
+    s = synth.config_system(cfg_idx)
+    ints = build_integrals(s.h, s.e2int(), s.enuc, 0, s.nocc, s.nocc)
+    return s, ints
+
+
+def run_iterations(eng, nsteps, diis_space, energies=None, stamps=None):
+    en_cor = 0.0
+    for it in range(1, nsteps + 1):
+        eng.update()
+        e = eng.energy()
+        eng.diis_record(it)
+        eng.residuum(it)
+        if it % (diis_space + 1) == 0:
+            eng.diis_extrapolate()
+        if energies is not None:
+            energies.append(e)
+        if stamps is not None:
+            eng.sync()
+            stamps.append(time.perf_counter())
+        en_cor = e
+    return en_cor
+
+
+def converge(eng, diis_space, tol=1e-11, max_iter=200):
+    conv = [1.0, 1.0, 1.0]
+    en = 0.0
+    for it in range(1, max_iter + 1):
+        eng.update()
+        e = eng.energy()
+        conv = [conv[1], conv[2], en - e]
+        en = e
+        eng.diis_record(it)
+        eng.residuum(it)
+        if it % (diis_space + 1) == 0:
+            eng.diis_extrapolate()
+        if all(abs(c) < tol for c in conv):
+            return en, it
+    return en, max_iter
+
+
+def cpu_baseline(s, ints, n_iter_needed, budget_s=25.0):
+    """Oracle (numpy, OpenBLAS threads) sweeps on the same system; extrapolated."""
+    from oracle import solver as osolver
+    o, n = s.nocc, s.norb
+    t = osolver.zero_amplitudes('CCSD', o, o, n - o, n - o)
+    prov = osolver.CCSDResiduals(s.h, ints.e2int, 0, o, o)
+    D = osolver.block_denominators(t, ints.f_a, ints.f_b, o, o, 0.0)
+    seq = osolver.sweep_order('CCSD', False)
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        osolver.sweep(t, seq, prov, D)
+        osolver.energy(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb, o, o, t)
+        done += 1
+        if time.perf_counter() - t0 > budget_s or done >= 3:
+            break
+    per_iter = (time.perf_counter() - t0) / done
+    threads = int(os.environ.get("OPENBLAS_NUM_THREADS") or os.environ.get("OMP_NUM_THREADS")
+                  or os.cpu_count() or 1)
+    return dict(value=per_iter * n_iter_needed, unit="s", cores=threads, kind="port",
+                sample=f"{done} oracle CCSD sweep(s)+energy of the same system "
+                       f"({per_iter:.2f} s each, numpy/OpenBLAS), x {n_iter_needed} iterations")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rhf", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
+                                   "only (see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+    diis = 5
+    t0 = time.perf_counter()
+    s, ints = make_workload(args.config)
+    o, n = s.nocc, s.norb
+    log(f"workload built in {time.perf_counter() - t0:.1f}s: n={n} o={o} v={n - o}")
+    eng = E.Engine(0, o, o, n, "CCSD", rhf=args.rhf, diis_space=diis)
+    t0 = time.perf_counter()
+    eng.set_ints(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb)
+    log(f"upload {time.perf_counter() - t0:.1f}s")
+    zero = np.zeros(eng.t_size)
+    # warm-up and the converged FP64 reference energy (untimed)
+    eng.set_t(zero)
+    run_iterations(eng, args.warmup, diis)
+    eng.set_t(zero)
+    e_conv, it_conv = converge(eng, diis)
+    log(f"E_conv = {e_conv:.12f} after {it_conv} iterations")
+    # timed region
+    eng.set_t(zero)
+    energies, stamps = [], []
+    eng.sync()
+    start = time.perf_counter()
+    run_iterations(eng, args.steps, diis, energies, stamps)
+    eng.sync()
+    total = time.perf_counter() - start
+    hit = next((i for i, e in enumerate(energies) if abs(e - e_conv) < 1e-7), None)
+    value = (stamps[hit] - start) if hit is not None else None
+    n_needed = hit + 1 if hit is not None else None
+    log(f"timed {args.steps} iterations: {total * 1e3 / args.steps:.3f} ms/iter; "
+        f"|dE|<1e-7 at iteration {n_needed}: {value}")
+    # roofline of the dominant kernel: one profiled sweep (HIP events on the stream)
+    eng.profile(True)
+    eng.update()
+    rep = eng.profile_report()
+    eng.profile(False)
+    gemms = {k: v for k, v in rep.items() if k.startswith("gemm:")}
+    dom = max(gemms.items(), key=lambda kv: kv[1]["ms"])
+    dom_name, d = dom
+    avg_s = d["ms"] / d["calls"] * 1e-3
+    fl_per = d["flops"] / d["calls"]
+    achieved = fl_per / avg_s
+    sweep_ms = sum(v["ms"] for v in rep.values())
+    gemm_ms = sum(v["ms"] for v in gemms.values())
+    gemm_fl = sum(v["flops"] for v in gemms.values())
+    log(f"dominant {dom_name}: {d['calls']} launches, avg {avg_s * 1e6:.1f} us, "
+        f"{achieved / 1e12:.2f} TF/s; all GEMMs {gemm_fl / 1e9:.1f} GF in {gemm_ms:.2f} ms "
+        f"({gemm_fl / (gemm_ms * 1e-3) / 1e12:.2f} TF/s); profiled sweep {sweep_ms:.2f} ms")
+    top = sorted(rep.items(), key=lambda kv: -kv[1]["ms"])[:8]
+    for k, v in top:
+        log(f"  {k:32s} calls={v['calls']:4d} ms={v['ms']:.3f}")
+    cpu = None
+    if not args.no_cpu and n_needed is not None:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(s, ints, n_needed)
+        log(f"cpu baseline {cpu['value']:.1f} s ({cpu['sample']})")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total * 1e3 / args.steps,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded integrals, ccq onebody/twobody construction)",
+        "config": {"workload": f"configs[{args.config}] CCSD plain FP64, n={n} o={o} v={n - o}, "
+                               f"L={synth.CONFIGS[args.config]['nL']}, seed "
+                               f"{synth.CONFIGS[args.config]['seed']}, diis 5, "
+                               f"{'rhf' if args.rhf else 'uhf'} sweep",
+                   "iterations_to_1e-7": n_needed, "e_conv": e_conv,
+                   "mode": "plain FP64 (mixed precision and learned guess not built yet)"},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                     "kernel": dom_name, "avg_launch_us": avg_s * 1e6,
+                     "flops_per_launch": fl_per,
+                     "all_gemm_tflops": gemm_fl / (gemm_ms * 1e-3) / 1e12},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

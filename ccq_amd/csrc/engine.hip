@@ -37,6 +37,7 @@ struct Fail : std::runtime_error {
   } while (0)
 
 constexpr int kPartialBlocks = 240;
+constexpr int64_t kWsCap = 16 << 20;   // split-K workspace bound (doubles)
 
 // ---------------------------------------------------------------------------
 // device buffers
@@ -67,6 +68,10 @@ struct Tensor {
   double* ptr = nullptr;
   std::string base;          // View: base array; Amp/Resid: block name
   std::string spec[8];       // View: per-index "oa", "vb", ...
+  // View: where the block lives inside the full-range base array; the view is
+  // gathered into a dense C-order copy (ptr/strides) after every upload
+  double* src = nullptr;
+  int64_t src_strides[8] = {0};
 };
 
 struct Op {
@@ -78,6 +83,8 @@ struct Op {
   int64_t M = 1, N = 1, K = 1;
   int64_t t_rowA = 0, t_kA = 0, t_kB = 0, t_colB = 0, t_rowC = 0, t_colC = 0;
   int a_kfast = 1, b_kfast = 1;
+  int bm = 64, bn = 64, nsplit = 1;
+  int64_t kchunk = 0;
   double flops = 0, bytes = 0;
   std::string cls;           // profile class
 };
@@ -130,6 +137,12 @@ struct ccq_hip_ctx {
   std::map<std::string, std::unique_ptr<DevBuf>> resid_buf;
   std::vector<std::unique_ptr<DevBuf>> inter_buf;
   int64_t* tables = nullptr;
+  int* tables32 = nullptr;   // GEMM offset tables
+  int64_t* gather_tab = nullptr;
+  std::vector<std::unique_ptr<DevBuf>> view_buf;
+  struct Gather { int t; int64_t rows, cols, t_src_row, t_src_col, t_dst_row, t_dst_col; };
+  std::vector<Gather> gathers;
+  DevBuf ws;                 // split-K workspace
   int* itab = nullptr;
   double* dtab = nullptr;
   // program
@@ -144,6 +157,8 @@ struct ccq_hip_ctx {
 
   ~ccq_hip_ctx() {
     if (tables) (void)hipFree(tables);
+    if (tables32) (void)hipFree(tables32);
+    if (gather_tab) (void)hipFree(gather_tab);
     if (itab) (void)hipFree(itab);
     if (dtab) (void)hipFree(dtab);
     if (stream) (void)hipStreamDestroy(stream);
@@ -309,7 +324,7 @@ struct ccq_hip_ctx {
           t.ptr = rb->p;
         }
       } else {
-        // view into a full-range base array
+        // view into a full-range base array, materialised densely
         double* basep = nullptr;
         int64_t nb = 0;
         if (t.base == "Vaa") basep = Vaa.p, nb = 4;
@@ -322,7 +337,7 @@ struct ccq_hip_ctx {
         int64_t s = 1;
         int64_t off = 0;
         for (int i = t.rank - 1; i >= 0; --i) {
-          t.strides[i] = s;
+          t.src_strides[i] = s;
           const char sp = t.spec[i][0];
           const bool beta = t.spec[i][1] == 'b';
           const int64_t occ = beta ? ob : oa;
@@ -330,9 +345,76 @@ struct ccq_hip_ctx {
           if (sp == 'v') off += occ * s;
           s *= nc;
         }
-        t.ptr = basep + off;
+        t.src = basep + off;
+        // dense copy shared by identical views
+        std::string key = std::to_string((uintptr_t)basep);
+        for (int i = 0; i < t.rank; ++i) key += t.spec[i];
+        auto hit = view_key.find(key);
+        int64_t d = 1;
+        for (int i = t.rank - 1; i >= 0; --i) {
+          t.strides[i] = d;
+          d *= t.dims[i];
+        }
+        if (hit != view_key.end()) {
+          t.ptr = hit->second;
+        } else {
+          view_buf.emplace_back(new DevBuf);
+          view_buf.back()->alloc(std::max<int64_t>(d, 1));
+          t.ptr = view_buf.back()->p;
+          view_key[key] = t.ptr;
+          view_owner.push_back((int)(&t - &tensors[0]));
+        }
       }
     }
+    plan_gathers();
+  }
+
+  std::map<std::string, double*> view_key;
+  std::vector<int> view_owner;
+
+  // tables that copy each distinct view out of its base array
+  void plan_gathers() {
+    std::vector<int64_t> host;
+    for (int ti : view_owner) {
+      const Tensor& t = tensors[ti];
+      std::vector<int64_t> ext(t.dims, t.dims + t.rank);
+      Gather g{};
+      g.t = ti;
+      g.cols = ext.back();
+      g.rows = 1;
+      for (int i = 0; i + 1 < t.rank; ++i) g.rows *= ext[i];
+      std::vector<int64_t> sr((size_t)g.rows), dr((size_t)g.rows), sc((size_t)g.cols), dc((size_t)g.cols);
+      std::vector<int64_t> idx(t.rank > 1 ? t.rank - 1 : 1, 0);
+      for (int64_t r = 0; r < g.rows; ++r) {
+        int64_t so = 0, dof = 0;
+        for (int i = 0; i + 1 < t.rank; ++i) so += idx[i] * t.src_strides[i], dof += idx[i] * t.strides[i];
+        sr[r] = so;
+        dr[r] = dof;
+        for (int i = t.rank - 2; i >= 0; --i) {
+          if (++idx[i] < ext[i]) break;
+          idx[i] = 0;
+        }
+      }
+      for (int64_t c = 0; c < g.cols; ++c) sc[c] = c * t.src_strides[t.rank - 1], dc[c] = c * t.strides[t.rank - 1];
+      g.t_src_row = (int64_t)host.size(); host.insert(host.end(), sr.begin(), sr.end());
+      g.t_dst_row = (int64_t)host.size(); host.insert(host.end(), dr.begin(), dr.end());
+      g.t_src_col = (int64_t)host.size(); host.insert(host.end(), sc.begin(), sc.end());
+      g.t_dst_col = (int64_t)host.size(); host.insert(host.end(), dc.begin(), dc.end());
+      gathers.push_back(g);
+    }
+    HIPCHK(hipMalloc(&gather_tab, sizeof(int64_t) * std::max<size_t>(host.size(), 1)));
+    if (!host.empty())
+      HIPCHK(hipMemcpy(gather_tab, host.data(), sizeof(int64_t) * host.size(), hipMemcpyHostToDevice));
+  }
+
+  void gather_views() {
+    for (auto& g : gathers) {
+      const Tensor& t = tensors[g.t];
+      HIPCHK(ccqk::launch_perm_add(stream, g.rows, g.cols, t.src, gather_tab + g.t_src_row,
+                                   gather_tab + g.t_src_col, t.ptr, gather_tab + g.t_dst_row,
+                                   gather_tab + g.t_dst_col, 1.0, 0));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
   }
 
   static std::vector<int64_t> offsets(const Tensor& t, const std::string& labels,
@@ -366,8 +448,19 @@ struct ccq_hip_ctx {
     return t.strides[pos];
   }
 
+  int64_t ws_need = 0;
+
   void plan() {
     std::vector<int64_t> host;
+    std::vector<int> host32;
+    auto push32 = [&](const std::vector<int64_t>& v) {
+      int64_t o = (int64_t)host32.size();
+      for (auto x : v) {
+        if (x < 0 || x > 0x7fffffffLL) throw Fail("tensor offset exceeds int32 GEMM tables");
+        host32.push_back((int)x);
+      }
+      return o;
+    };
     std::vector<int> ih;
     std::vector<double> dh;
     auto push = [&](const std::vector<int64_t>& v) {
@@ -426,20 +519,33 @@ struct ccq_hip_ctx {
           op.M = (int64_t)rA.size();
           op.N = (int64_t)cB.size();
           op.K = (int64_t)kA.size();
-          op.t_rowA = push(rA);
-          op.t_kA = push(kA);
-          op.t_kB = push(kB);
-          op.t_colB = push(cB);
-          op.t_rowC = push(rC);
-          op.t_colC = push(cC);
+          op.t_rowA = push32(rA);
+          op.t_kA = push32(kA);
+          op.t_kB = push32(kB);
+          op.t_colB = push32(cB);
+          op.t_rowC = push32(rC);
+          op.t_colC = push32(cC);
           op.a_kfast = ks.empty() ? 1 : (stride_of(A, op.la, ks.back()) == 1 ? 1
                       : (!ms.empty() && stride_of(A, op.la, ms.back()) == 1 ? 0 : 1));
           op.b_kfast = ks.empty() ? 1 : (stride_of(B, op.lb, ks.back()) == 1 ? 1
                       : (!ns.empty() && stride_of(B, op.lb, ns.back()) == 1 ? 0 : 1));
+          {
+            ccqk::GemmArgs g{};
+            g.M = op.M;
+            g.N = op.N;
+            g.K = op.K;
+            ccqk::gemm_plan(g, kWsCap);
+            op.bm = g.bm;
+            op.bn = g.bn;
+            op.nsplit = g.nsplit;
+            op.kchunk = g.kchunk;
+            if (g.nsplit > 1) ws_need = std::max<int64_t>(ws_need, (int64_t)g.nsplit * op.M * op.N);
+          }
+          op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +
+                   std::to_string(op.K) + "/" + std::to_string(op.bm) + "x" + std::to_string(op.bn) +
+                   "s" + std::to_string(op.nsplit);
           op.flops = 2.0 * (double)op.M * (double)op.N * (double)(ks.empty() ? 1 : op.K);
           op.bytes = 8.0 * ((double)op.M * op.K + (double)op.K * op.N + 2.0 * op.M * op.N);
-          op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +
-                   std::to_string(op.K);
         } else {
           if (op.la.size() != op.lc.size()) throw Fail("add op must permute its input");
           ms = op.lc.substr(0, op.lc.size() - 1);
@@ -461,6 +567,10 @@ struct ccq_hip_ctx {
     HIPCHK(hipMalloc(&tables, sizeof(int64_t) * std::max<size_t>(host.size(), 1)));
     if (!host.empty())
       HIPCHK(hipMemcpy(tables, host.data(), sizeof(int64_t) * host.size(), hipMemcpyHostToDevice));
+    ws.alloc(std::max<int64_t>(ws_need, 1));
+    HIPCHK(hipMalloc(&tables32, sizeof(int) * std::max<size_t>(host32.size(), 1)));
+    if (!host32.empty())
+      HIPCHK(hipMemcpy(tables32, host32.data(), sizeof(int) * host32.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&itab, sizeof(int) * std::max<size_t>(ih.size(), 1)));
     if (!ih.empty()) HIPCHK(hipMemcpy(itab, ih.data(), sizeof(int) * ih.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMalloc(&dtab, sizeof(double) * std::max<size_t>(dh.size(), 1)));
@@ -482,19 +592,23 @@ struct ccq_hip_ctx {
       g.N = op.N;
       g.K = op.K;
       g.A = A.ptr;
-      g.rowA = tables + op.t_rowA;
-      g.kA = tables + op.t_kA;
+      g.rowA = tables32 + op.t_rowA;
+      g.kA = tables32 + op.t_kA;
       g.B = B.ptr;
-      g.kB = tables + op.t_kB;
-      g.colB = tables + op.t_colB;
+      g.kB = tables32 + op.t_kB;
+      g.colB = tables32 + op.t_colB;
       g.C = C.ptr;
-      g.rowC = tables + op.t_rowC;
-      g.colC = tables + op.t_colC;
+      g.rowC = tables32 + op.t_rowC;
+      g.colC = tables32 + op.t_colC;
       g.alpha = op.coef;
       g.accumulate = acc;
       g.a_kfast = op.a_kfast;
       g.b_kfast = op.b_kfast;
-      HIPCHK(ccqk::launch_gemm(stream, g));
+      g.bm = op.bm;
+      g.bn = op.bn;
+      g.nsplit = op.nsplit;
+      g.kchunk = op.kchunk;
+      HIPCHK(ccqk::launch_gemm(stream, g, ws.p));
     } else {
       HIPCHK(ccqk::launch_perm_add(stream, op.M, op.N, A.ptr, tables + op.t_rowA, tables + op.t_kA,
                                    C.ptr, tables + op.t_rowC, tables + op.t_colC, op.coef, acc));
@@ -795,6 +909,7 @@ int ccq_hip_set_ints(ccq_hip_ctx* ctx, const double* f_a, const double* f_b, con
       ctx->resolve_tensors();
       ctx->plan();
     }
+    ctx->gather_views();
     ctx->ints_ready = true;
   });
 }

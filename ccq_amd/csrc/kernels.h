@@ -8,21 +8,27 @@ namespace ccqk {
 constexpr int kMaxRank = 8;
 constexpr int kMaxDiis = 12;
 
+// Offset tables are int32: every tensor the engine addresses holds < 2^31
+// elements (checked when the tables are built).
 struct GemmArgs {
   int64_t M, N, K;
   const double* A;
-  const int64_t* rowA;
-  const int64_t* kA;
+  const int* rowA;
+  const int* kA;
   const double* B;
-  const int64_t* kB;
-  const int64_t* colB;
+  const int* kB;
+  const int* colB;
   double* C;
-  const int64_t* rowC;
-  const int64_t* colC;
+  const int* rowC;
+  const int* colC;
   double alpha;
   int accumulate;
   int a_kfast;
   int b_kfast;
+  // launch plan (gemm_plan)
+  int bm = 64, bn = 64;
+  int nsplit = 1;
+  int64_t kchunk = 0;
 };
 
 struct UpdateParams {
@@ -53,7 +59,10 @@ struct Coeffs {
   double c[kMaxDiis];
 };
 
-hipError_t launch_gemm(hipStream_t s, const GemmArgs& g);
+// Choose tile shape and split-K for a GEMM; ws_cap bounds nsplit*M*N.
+void gemm_plan(GemmArgs& g, int64_t ws_cap);
+// ws: split-K workspace of at least nsplit*M*N doubles (unused when nsplit == 1).
+hipError_t launch_gemm(hipStream_t s, const GemmArgs& g, double* ws);
 hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A,
                            const int64_t* rowA, const int64_t* colA, double* C,
                            const int64_t* rowC, const int64_t* colC, double alpha, int accumulate);

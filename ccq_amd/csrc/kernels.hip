@@ -16,93 +16,175 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "kernels.h"
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace ccqk {
 
-constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int BK = 16;
 
-__global__ __launch_bounds__(256) void gemm_f64_tab(
-    int64_t M, int64_t N, int64_t K,
-    const double* __restrict__ A, const int64_t* __restrict__ rowA, const int64_t* __restrict__ kA,
-    const double* __restrict__ B, const int64_t* __restrict__ kB, const int64_t* __restrict__ colB,
-    double* __restrict__ C, const int64_t* __restrict__ rowC, const int64_t* __restrict__ colC,
-    double alpha, int accumulate, int a_kfast, int b_kfast) {
-  __shared__ double As[BK][BM];
-  __shared__ double Bs[BK][BN];
+// LDS tile of an operand: R x BK values.  KF (k-fastest in global memory):
+// stored row-major [R][BK+1] so a 16-lane group writing consecutive k and
+// the MFMA fragment read (16 consecutive rows, one k) are both bank-conflict
+// free.  Otherwise stored k-major [BK][R+2].
+template <int R, bool KF>
+struct LdsTile {
+  static constexpr int size = KF ? R * (BK + 1) : BK * (R + 2);
+  __device__ static int at(int row, int k) { return KF ? row * (BK + 1) + k : k * (R + 2) + row; }
+};
+
+// Per-thread loader of one operand slab (R rows x BK k-values per slab).
+// KF: thread owns k = tid % BK and rows tid / BK + j * (256 / BK);
+// else: thread owns row tid % R and k = tid / R + j * (256 / R).
+template <int R, bool KF, bool ISB>
+struct Loader {
+  static constexpr int L = R * BK / 256;
+  int row_off[L];        // table offset of each owned row (or -1)
+  int kk[L];             // owned k within slab
+  int rr[L];             // owned row within tile
+  __device__ void init(const int* __restrict__ rowtab, int64_t r0, int64_t R_total, int tid) {
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+      int row, k;
+      if (KF) { k = tid % BK; row = tid / BK + j * (256 / BK); }
+      else { row = tid % R; k = tid / R + j * (256 / R); }
+      kk[j] = k;
+      rr[j] = row;
+      const int64_t g = r0 + row;
+      row_off[j] = g < R_total ? rowtab[g] : -1;
+    }
+  }
+  __device__ void fetch(double (&reg)[L], const double* __restrict__ X, const int* __restrict__ ktab,
+                        int64_t k0, int64_t kend) const {
+    if (KF) {
+      const int64_t gk = k0 + kk[0];
+      const bool kin = gk < kend;
+      const int ko = kin ? ktab[gk] : 0;
+#pragma unroll
+      for (int j = 0; j < L; ++j) reg[j] = (kin && row_off[j] >= 0) ? X[(int64_t)row_off[j] + ko] : 0.0;
+    } else {
+#pragma unroll
+      for (int j = 0; j < L; ++j) {
+        const int64_t gk = k0 + kk[j];
+        reg[j] = (gk < kend && row_off[j] >= 0) ? X[(int64_t)row_off[j] + ktab[gk]] : 0.0;
+      }
+    }
+  }
+  __device__ void stash(double* lds, const double (&reg)[L]) const {
+#pragma unroll
+    for (int j = 0; j < L; ++j) lds[LdsTile<R, KF>::at(rr[j], kk[j])] = reg[j];
+  }
+};
+
+// Tile GEMM C(M x N) (+)= alpha * A(M x K) B(K x N) on v_mfma_f64_16x16x4f64.
+// 256 threads = 4 waves in a 2 x 2 grid; each wave owns a (BM/2) x (BN/2)
+// sub-tile of 16x16 MFMA tiles.  Operands are gathered through int32 offset
+// tables (rowA/kA, kB/colB); the next K slab is fetched into registers while
+// the current one is multiplied out of LDS.
+// Split-K: blockIdx.z owns K range [z*kchunk, (z+1)*kchunk); with nsplit > 1
+// the tile goes to ws[z][m][n] and gemm_splitk_reduce finishes it.
+template <int BM, int BN, bool AKF, bool BKF>
+__global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g, int64_t kchunk, double* __restrict__ ws) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  using TA = LdsTile<BM, AKF>;
+  using TB = LdsTile<BN, BKF>;
+  __shared__ double As[TA::size];
+  __shared__ double Bs[TB::size];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int64_t m0 = (int64_t)blockIdx.x * BM;
   const int64_t n0 = (int64_t)blockIdx.y * BN;
-  const int wm = (wave & 1) * 32;
-  const int wn = (wave >> 1) * 32;
+  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kend = (kbeg + kchunk < g.K) ? kbeg + kchunk : g.K;
+  const int wm = (wave & 1) * WM;
+  const int wn = (wave >> 1) * WN;
 
-  // per-thread load coordinates (fixed across the k loop)
-  int am[4], ak[4], bn[4], bk[4];
-  int64_t arow[4], bcol[4];
+  Loader<BM, AKF, false> la;
+  Loader<BN, BKF, true> lb;
+  la.init(g.rowA, m0, g.M, tid);
+  lb.init(g.colB, n0, g.N, tid);
+  double ra[Loader<BM, AKF, false>::L], rb[Loader<BN, BKF, true>::L];
+
+  d4 acc[TM][TN];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int e = tid + 256 * r;
-    if (a_kfast) { ak[r] = e & 15; am[r] = e >> 4; } else { am[r] = e & 63; ak[r] = e >> 6; }
-    if (b_kfast) { bk[r] = e & 15; bn[r] = e >> 4; } else { bn[r] = e & 63; bk[r] = e >> 6; }
-    const int64_t gm = m0 + am[r];
-    const int64_t gn = n0 + bn[r];
-    arow[r] = gm < M ? rowA[gm] : -1;
-    bcol[r] = gn < N ? colB[gn] : -1;
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+
+  const int64_t nsteps = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nsteps > 0) {
+    la.fetch(ra, g.A, g.kA, kbeg, kend);
+    lb.fetch(rb, g.B, g.kB, kbeg, kend);
   }
-
-  d4 acc00 = {0, 0, 0, 0}, acc01 = {0, 0, 0, 0}, acc10 = {0, 0, 0, 0}, acc11 = {0, 0, 0, 0};
-  for (int64_t k0 = 0; k0 < K; k0 += BK) {
-    double ra[4], rb[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t gka = k0 + ak[r];
-      ra[r] = (arow[r] >= 0 && gka < K) ? A[arow[r] + kA[gka]] : 0.0;
-      const int64_t gkb = k0 + bk[r];
-      rb[r] = (bcol[r] >= 0 && gkb < K) ? B[kB[gkb] + bcol[r]] : 0.0;
-    }
+  for (int64_t s = 0; s < nsteps; ++s) {
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      As[ak[r]][am[r]] = ra[r];
-      Bs[bk[r]][bn[r]] = rb[r];
-    }
+    la.stash(As, ra);
+    lb.stash(Bs, rb);
     __syncthreads();
+    if (s + 1 < nsteps) {
+      la.fetch(ra, g.A, g.kA, kbeg + (s + 1) * BK, kend);
+      lb.fetch(rb, g.B, g.kB, kbeg + (s + 1) * BK, kend);
+    }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4);
       const int rr = lane & 15;
-      const double a0 = As[kr][wm + rr];
-      const double a1 = As[kr][wm + 16 + rr];
-      const double b0 = Bs[kr][wn + rr];
-      const double b1 = Bs[kr][wn + 16 + rr];
-      acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-      acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-      acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-      acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+      double af[TM], bf[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[i] = As[TA::at(wm + 16 * i + rr, kr)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[TB::at(wn + 16 * j + rr, kr)];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
     }
   }
+
   // D layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l, register v
   // holds row (l/16) + 4*v, column l%16
-  const d4* accs[4] = {&acc00, &acc01, &acc10, &acc11};
+  const int64_t M = g.M, N = g.N;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int i = t >> 1, j = t & 1;
+  for (int j = 0; j < TN; ++j) {
     const int64_t gn = n0 + wn + 16 * j + (lane & 15);
     if (gn >= N) continue;
-    const int64_t cc = colC[gn];
+    const int64_t cc = ws ? gn : (int64_t)g.colC[gn];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t gm = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
-      if (gm < M) {
-        double* p = C + rowC[gm] + cc;
-        const double val = alpha * (*accs[t])[v];
-        if (accumulate) *p += val; else *p = val;
+    for (int i = 0; i < TM; ++i) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gm = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
+        if (gm >= M) continue;
+        if (ws) {
+          ws[((int64_t)blockIdx.z * M + gm) * N + cc] = acc[i][j][v];
+        } else {
+          double* p = g.C + (int64_t)g.rowC[gm] + cc;
+          const double val = g.alpha * acc[i][j][v];
+          if (g.accumulate) *p += val; else *p = val;
+        }
       }
     }
+  }
+}
+
+// C = beta C + alpha * sum_z ws[z] in fixed z order (deterministic).
+__global__ void gemm_splitk_reduce(GemmArgs g, int nsplit, const double* __restrict__ ws) {
+  const int64_t total = g.M * g.N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int z = 0; z < nsplit; ++z) s += ws[(int64_t)z * total + e];
+    const int64_t m = e / g.N, n = e - m * g.N;
+    double* p = g.C + (int64_t)g.rowC[m] + g.colC[n];
+    const double val = g.alpha * s;
+    if (g.accumulate) *p += val; else *p = val;
   }
 }
 
@@ -295,11 +377,67 @@ static int grid_for(int64_t n, int threads, int cap = 4096) {
   return (int)g;
 }
 
-hipError_t launch_gemm(hipStream_t s, const GemmArgs& g) {
-  dim3 grid((unsigned)((g.M + BM - 1) / BM), (unsigned)((g.N + BN - 1) / BN), 1);
-  hipLaunchKernelGGL(gemm_f64_tab, grid, dim3(256), 0, s, g.M, g.N, g.K, g.A, g.rowA, g.kA, g.B,
-                     g.kB, g.colB, g.C, g.rowC, g.colC, g.alpha, g.accumulate, g.a_kfast,
-                     g.b_kfast);
+void gemm_plan(GemmArgs& g, int64_t ws_cap) {
+  // Cost model in MFMA cycles of the busiest SIMD.  A 16x16x4 FP64 MFMA
+  // occupies a SIMD for 64 cycles (78.6 TF/s over 256 CUs x 4 SIMDs at
+  // 2.4 GHz); a WG of 4 waves keeps each SIMD busy for (BM/32)(BN/32)*4 MFMAs
+  // per 16-deep K slab.  Resident WGs per CU (from VGPR/LDS use): 2 for
+  // 128x128, 3 for 128x64 / 64x128, 4 for 64x64.  Split-K adds the
+  // workspace round trip and a reduction pass.
+  struct Shape { int bm, bn, occ; };
+  const Shape shapes[4] = {{128, 128, 2}, {128, 64, 3}, {64, 128, 3}, {64, 64, 4}};
+  const double cus = 256.0;
+  const int64_t ksteps = (g.K + BK - 1) / BK;
+  double best = 1e300;
+  for (const Shape& sh : shapes) {
+    const int64_t tiles = ((g.M + sh.bm - 1) / sh.bm) * ((g.N + sh.bn - 1) / sh.bn);
+    const double slab = (sh.bm / 32.0) * (sh.bn / 32.0) * 4.0 * 64.0;   // cycles per K slab
+    const int64_t maxsplit = std::max<int64_t>(1, std::min<int64_t>(ksteps / 2, 4096));
+    for (int64_t split = 1; split <= maxsplit; split *= 2) {
+      if (split > 1 && split * g.M * g.N > ws_cap) break;
+      const int64_t steps = (ksteps + split - 1) / split;
+      // MFMA work is balanced over CUs in whole WGs; a CU holding a single WG
+      // cannot hide the global-load latency of a slab (~1500 cycles).
+      const double per_cu = std::ceil((double)tiles * split / cus);
+      const double resident = std::min<double>(per_cu, sh.occ);
+      double cyc = per_cu * (steps + 3) * slab;
+      if (resident < 2) cyc += per_cu * steps * 1500.0;
+      if (split > 1) cyc += (2.0 * split * g.M * g.N * 8.0) / 8.0e12 * 2.4e9 + 12000.0;
+      if (cyc < best) {
+        best = cyc;
+        g.bm = sh.bm;
+        g.bn = sh.bn;
+        g.nsplit = (int)split;
+        g.kchunk = steps * BK;
+      }
+    }
+  }
+}
+
+template <int BM, int BN>
+static void launch_tile(hipStream_t s, dim3 grid, const GemmArgs& g, double* w) {
+  if (g.a_kfast && g.b_kfast)
+    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, true, true>), grid, dim3(256), 0, s, g, g.kchunk, w);
+  else if (g.a_kfast)
+    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, true, false>), grid, dim3(256), 0, s, g, g.kchunk, w);
+  else if (g.b_kfast)
+    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, false, true>), grid, dim3(256), 0, s, g, g.kchunk, w);
+  else
+    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, false, false>), grid, dim3(256), 0, s, g, g.kchunk, w);
+}
+
+hipError_t launch_gemm(hipStream_t s, const GemmArgs& g, double* ws) {
+  dim3 grid((unsigned)((g.M + g.bm - 1) / g.bm), (unsigned)((g.N + g.bn - 1) / g.bn),
+            (unsigned)g.nsplit);
+  double* w = g.nsplit > 1 ? ws : nullptr;
+  if (g.bm == 128 && g.bn == 128) launch_tile<128, 128>(s, grid, g, w);
+  else if (g.bm == 128) launch_tile<128, 64>(s, grid, g, w);
+  else if (g.bn == 128) launch_tile<64, 128>(s, grid, g, w);
+  else launch_tile<64, 64>(s, grid, g, w);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || g.nsplit == 1) return e;
+  hipLaunchKernelGGL(gemm_splitk_reduce, dim3(grid_for(g.M * g.N, 256)), dim3(256), 0, s, g,
+                     g.nsplit, (const double*)ws);
   return hipGetLastError();
 }
 
