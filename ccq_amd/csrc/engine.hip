@@ -22,6 +22,8 @@
 
 #include "ir_ccsd_rhf.inc"
 #include "ir_ccsd_uhf.inc"
+#include "ir_ccsdt_rhf.inc"
+#include "ir_ccsdt_uhf.inc"
 
 namespace {
 
@@ -625,7 +627,6 @@ struct ccq_hip_ctx {
     // positions: first half occupied, second half virtual; spin pattern by block name
     const int rank = R.rank, half = rank / 2;
     std::string spins = b.substr(2);  // e.g. "aab" from t3aab, "a" from t1a
-    if (spins.size() == 1 && rank == 2) spins = spins;  // t1a -> "a"
     for (int d = 0; d < rank; ++d) {
       P.dims[d] = R.dims[d];
       P.strides[d] = R.strides[d];
@@ -829,7 +830,8 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (ctx->oa < 0 || ctx->ob < 0 || ctx->va < 0 || ctx->vb < 0 || ctx->nc <= 0)
       throw Fail("inconsistent orbital counts");
     ctx->level = (int)opts->level;
-    if (ctx->level != 2) throw Fail("only CCSD (level 2) is built into this library");
+    if (ctx->level != 2 && ctx->level != 3)
+      throw Fail("this library implements CCSD (level 2) and CCSDT (level 3)");
     ctx->rhf = opts->rhf != 0;
     if (ctx->rhf && ctx->oa != ctx->ob) throw Fail("rhf requires occ_a == occ_b");
     ctx->shift = opts->shift;
@@ -850,7 +852,8 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     HIPCHK(hipMemsetAsync(ctx->tvec.p, 0, sizeof(double) * ctx->dev_t, ctx->stream));
     ctx->ring.alloc((int64_t)(ctx->diis + 1) * ctx->vec_size);
     ctx->partial.alloc(kPartialBlocks * 128);
-    ctx->load_ir(ctx->rhf ? kIR_CCSD_RHF : kIR_CCSD_UHF);
+    if (ctx->level == 2) ctx->load_ir(ctx->rhf ? kIR_CCSD_RHF : kIR_CCSD_UHF);
+    else ctx->load_ir(ctx->rhf ? kIR_CCSDT_RHF : kIR_CCSDT_UHF);
     HIPCHK(hipStreamSynchronize(ctx->stream));
   });
   if (rc != 0) {

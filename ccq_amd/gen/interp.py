@@ -17,9 +17,11 @@ AMP_BLOCKS = {
     ('T2', (0, 0, 0, 0)): 't2aa', ('T2', (0, 1, 0, 1)): 't2ab', ('T2', (1, 1, 1, 1)): 't2bb',
     ('T3', (0, 0, 0, 0, 0, 0)): 't3aaa', ('T3', (0, 0, 1, 0, 0, 1)): 't3aab',
     ('T3', (0, 1, 1, 0, 1, 1)): 't3abb', ('T3', (1, 1, 1, 1, 1, 1)): 't3bbb',
+    ('T4', (0, 0, 0, 0, 0, 0, 0, 0)): 't4aaaa', ('T4', (0, 0, 0, 1, 0, 0, 0, 1)): 't4aaab',
+    ('T4', (0, 0, 1, 1, 0, 0, 1, 1)): 't4aabb', ('T4', (0, 1, 1, 1, 0, 1, 1, 1)): 't4abbb',
+    ('T4', (1, 1, 1, 1, 1, 1, 1, 1)): 't4bbbb',
 }
-RES_OF = {'t1a': ('R1', (0, 0)), 't1b': ('R1', (1, 1)),
-          't2aa': ('R2', (0, 0, 0, 0)), 't2ab': ('R2', (0, 1, 0, 1)), 't2bb': ('R2', (1, 1, 1, 1))}
+RES_OF = {v: ('R' + k[0][1:], k[1]) for k, v in AMP_BLOCKS.items()}
 
 
 class Dims:
@@ -86,6 +88,13 @@ class Interpreter:
                 spec = ",".join(l for _, l in op.ins) + "->" + op.out_labels
                 val = np.einsum(spec, *[self.get(k) for k, _ in op.ins], optimize=True)
                 val = float(op.coef) * val
+                if self.prog.decls[op.out.name].kind == 'amp':
+                    blk = self.amps[AMP_BLOCKS[(op.out.name, op.out.spins)]]
+                    if op.assign:
+                        blk[...] = val
+                    else:
+                        blk += val
+                    continue
                 if op.assign or op.out not in self.store:
                     self.store[op.out] = np.array(val, copy=True).reshape(out_shape)
                 else:
