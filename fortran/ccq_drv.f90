@@ -1,0 +1,216 @@
+! Minimal ccq-compatible driver for the GPU amplitude engine.
+!
+! Reads a ccq keyword file (src/parser.f90 semantics for the keywords used by
+! the CC iteration), the onebody/twobody integral files (src/integrals.f90:
+! 11-91), builds f_a, f_b, v_aa, v_ab, v_bb and E_HF exactly like ccq
+! (src/integrals.f90:212-364, src/energy.f90:49-101), then runs ccq's
+! jacobi_iter (src/solve_cc.f90:175-269) with every amplitude operation on
+! the GPU through ccq_hip_mod, printing ccq's iteration and summary lines
+! (src/printing.F90:303-349, 260-284).
+program ccq_drv
+
+    use, intrinsic :: iso_c_binding
+    use ccq_hip_mod
+
+    implicit none
+
+    integer, parameter :: dp = c_double
+    character(len=512) :: inp, line, key, val, onebody, twobody, calc_type
+    integer(c_int64_t) :: froz_spin, nel, nvirt, mult, froz, occ_a, occ_b, orbs
+    integer(c_int64_t) :: diis_space, max_iter, level, rhf_flag
+    real(dp) :: tol, shift
+    real(dp), allocatable :: e1int(:,:), e2int(:,:,:,:), f_a(:,:), f_b(:,:)
+    real(dp), allocatable :: v_aa(:,:,:,:), v_ab(:,:,:,:)
+    real(dp) :: en_repul, en_ref, e_cor_new, e_cor, diff, res, conv(3)
+    integer(c_int64_t) :: it, i, j, k, a, b, ios, unit_in, itol, eq
+    logical :: rhf, converged
+    type(ccq_hip_dims) :: dims
+    type(ccq_hip_opts) :: opts
+    type(c_ptr) :: ctx
+    real(dp) :: t0, t1
+
+    call get_command_argument(1, inp)
+    if (len_trim(inp) == 0) then
+        write(0, '(a)') 'usage: ccq_drv input.inp'
+        call exit(1)
+    end if
+
+    ! defaults (src/parser.f90:779-844)
+    froz_spin = 0; nel = -1; nvirt = -1; mult = 1
+    tol = 1.0e-4_dp; diis_space = 5; max_iter = 60; shift = 0.0_dp; rhf = .false.
+    onebody = 'onebody.inp'; twobody = 'twobody.inp'; calc_type = 'CCSD'
+
+    open(newunit=unit_in, file=trim(inp), status='old', action='read')
+    do
+        read(unit_in, '(a)', iostat=ios) line
+        if (ios /= 0) exit
+        if (index(line, '#') + index(line, '!') /= 0) cycle
+        if (len_trim(line) == 0) cycle
+        eq = index(line, '=')
+        if (eq == 0) cycle
+        key = adjustl(line(1:eq - 1))
+        val = adjustl(line(eq + 1:))
+        select case (trim(key))
+        case ('core', 'nfroz', 'frozen', 'froz'); read(val, *) froz_spin
+        case ('nel', 'electrons'); read(val, *) nel
+        case ('nvir', 'nvirt', 'virtuals'); read(val, *) nvirt
+        case ('multiplicity', 'mult'); read(val, *) mult
+        case ('rhf'); read(val, *) rhf
+        case ('shift'); read(val, *) shift
+        case ('max_iterations', 'max_iter'); read(val, *) max_iter
+        case ('diis_space', 'diis'); read(val, *) diis_space
+        case ('tolerance', 'tol', 'itol')
+            read(val, '(i10)', iostat=ios) itol
+            if (ios /= 0) then
+                read(val, *) tol
+            else
+                tol = 10.0_dp ** itol
+            end if
+        case ('onebody'); onebody = val
+        case ('twobody'); twobody = val
+        case ('calc_type'); calc_type = val
+        end select
+    end do
+    close(unit_in)
+
+    ! src/parser.f90:476-504
+    froz = froz_spin / 2
+    occ_a = nel / 2
+    occ_b = occ_a - (mult - 1) / 2
+    orbs = (nel + nvirt) / 2
+    select case (trim(calc_type))
+    case ('CCSD'); level = 2
+    case ('CCSDT'); level = 3
+    case default
+        write(0, '(a)') 'ccq_drv: calc_type '//trim(calc_type)//' is not supported'
+        call exit(1)
+    end select
+
+    allocate(e1int(orbs, orbs), e2int(orbs, orbs, orbs, orbs))
+    e1int = 0; e2int = 0; en_repul = 0
+    open(newunit=unit_in, file=trim(onebody), status='old', action='read')
+    do i = 1, orbs
+        do j = 1, i
+            read(unit_in, *) e1int(i, j)
+            e1int(j, i) = e1int(i, j)
+        end do
+    end do
+    close(unit_in)
+    open(newunit=unit_in, file=trim(twobody), status='old', action='read')
+    do
+        read(unit_in, *, iostat=ios) i, a, j, b, t0
+        if (ios /= 0) exit
+        if (i + a + j + b == 0) then
+            en_repul = t0
+            exit
+        end if
+        e2int(i, j, a, b) = t0
+    end do
+    close(unit_in)
+
+    allocate(v_aa(froz+1:orbs, froz+1:orbs, froz+1:orbs, froz+1:orbs))
+    allocate(v_ab(froz+1:orbs, froz+1:orbs, froz+1:orbs, froz+1:orbs))
+    do b = froz + 1, orbs
+        do a = froz + 1, orbs
+            do j = froz + 1, orbs
+                do i = froz + 1, orbs
+                    v_ab(i, j, a, b) = e2int(i, j, a, b)
+                    v_aa(i, j, a, b) = e2int(i, j, a, b) - e2int(i, j, b, a)
+                end do
+            end do
+        end do
+    end do
+    allocate(f_a(orbs, orbs), f_b(orbs, orbs))
+    f_a = e1int; f_b = e1int
+    do i = 1, orbs
+        do j = 1, i
+            do k = 1, occ_b
+                f_a(i, j) = f_a(i, j) + e2int(i, k, j, k)
+            end do
+            do k = 1, occ_a
+                f_a(i, j) = f_a(i, j) + (e2int(i, k, j, k) - e2int(i, k, k, j))
+                f_b(i, j) = f_b(i, j) + e2int(i, k, j, k)
+            end do
+            do k = 1, occ_b
+                f_b(i, j) = f_b(i, j) + (e2int(i, k, j, k) - e2int(i, k, k, j))
+            end do
+            f_a(j, i) = f_a(i, j)
+            f_b(j, i) = f_b(i, j)
+        end do
+    end do
+    en_ref = 0
+    do i = 1, occ_a
+        en_ref = en_ref + e1int(i, i)
+    end do
+    do i = 1, occ_b
+        en_ref = en_ref + e1int(i, i)
+    end do
+    do i = 1, occ_a
+        do j = 1, occ_b
+            en_ref = en_ref + e2int(i, j, i, j)
+        end do
+    end do
+    do i = 1, occ_a
+        do j = 1, occ_a
+            en_ref = en_ref + 0.5_dp * (e2int(i, j, i, j) - e2int(i, j, j, i))
+        end do
+    end do
+    do i = 1, occ_b
+        do j = 1, occ_b
+            en_ref = en_ref + 0.5_dp * (e2int(i, j, i, j) - e2int(i, j, j, i))
+        end do
+    end do
+    en_ref = en_ref + en_repul
+
+    dims = ccq_hip_dims(froz, occ_a, occ_b, orbs)
+    rhf_flag = merge(1_c_int64_t, 0_c_int64_t, rhf)
+    opts = ccq_hip_opts(level, rhf_flag, shift, diis_space, 0_c_int64_t)
+    if (ccq_hip_create(dims, opts, ctx) /= 0) then
+        write(0, '(a)') 'ERROR in ccq_hip_create'
+        call exit(1)
+    end if
+    call hip_check(ctx, ccq_hip_set_ints(ctx, f_a, f_b, v_aa, v_ab, v_aa), 'set_ints')
+
+    write(*, '(a)') 'Coupled cluster'
+    write(*, '(a)') '---------------'
+    write(*, '(/2x,a4,3(a15),a16)') 'It.', 'E (Corr)', 'dE', 'Residuum', 'Wall Time'
+    write(*, '(2x,65("-"))')
+    conv = 1.0_dp
+    e_cor = 0.0_dp
+    converged = .false.
+    call cpu_time(t0)
+    do it = 1, max_iter
+        call hip_check(ctx, ccq_hip_update(ctx), 'update')
+        call hip_check(ctx, ccq_hip_energy(ctx, e_cor_new), 'energy')
+        diff = e_cor - e_cor_new
+        conv(1) = conv(2); conv(2) = conv(3); conv(3) = diff
+        e_cor = e_cor_new
+        call hip_check(ctx, ccq_hip_diis_record(ctx, it), 'diis_record')
+        call hip_check(ctx, ccq_hip_residuum(ctx, it, res), 'residuum')
+        if (mod(it, diis_space + 1) == 0) then
+            write(*, '(a)') '      DIIS cycle'
+            call hip_check(ctx, ccq_hip_diis_extrapolate(ctx), 'calc_diis')
+        end if
+        call cpu_time(t1)
+        write(*, '(2x,i4,3(f15.10),i5,'' min'',f5.1,'' s'')') it, e_cor_new, diff, res, 0, t1 - t0
+        t0 = t1
+        if (all(abs(conv) < tol)) then
+            converged = .true.
+            exit
+        end if
+        if (e_cor_new /= e_cor_new) then
+            write(*, '(a)') 'FAILED TO CONVERGE'
+            exit
+        end if
+        if (it == max_iter) write(*, '(a)') 'FAILED TO CONVERGE IN', max_iter, &
+            ' ITERATIONS. KEEPING LAST T VECTOR.'
+    end do
+
+    write(*, '(/a)') 'Calculation Summary (Eh)'
+    write(*, '(a)') '------------------------'
+    write(*, '(a20,f18.12)') 'Reference', en_ref
+    write(*, '(a20,f18.12)') 'Correlation', e_cor
+    write(*, '(a20,f18.12)') trim(calc_type), en_ref + e_cor
+    i = ccq_hip_destroy(ctx)
+
+end program ccq_drv
