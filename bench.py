@@ -171,9 +171,9 @@ def main():
     log(f"dominant {dom_name}: {d['calls']} launches, avg {avg_s * 1e6:.1f} us, "
         f"{achieved / 1e12:.2f} TF/s; all GEMMs {gemm_fl / 1e9:.1f} GF in {gemm_ms:.2f} ms "
         f"({gemm_fl / (gemm_ms * 1e-3) / 1e12:.2f} TF/s); profiled sweep {sweep_ms:.2f} ms")
-    top = sorted(rep.items(), key=lambda kv: -kv[1]["ms"])[:8]
+    top = sorted(rep.items(), key=lambda kv: -kv[1]["ms"])[:16]
     for k, v in top:
-        log(f"  {k:32s} calls={v['calls']:4d} ms={v['ms']:.3f}")
+        log(f"  {k:36s} calls={v['calls']:4d} ms={v['ms']:.3f} avg_us={v['ms'] * 1e3 / v['calls']:.1f}")
     cpu = None
     if not args.no_cpu and n_needed is not None:
         log("cpu baseline (oracle) ...")

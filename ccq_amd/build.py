@@ -24,14 +24,14 @@ FLAGS = [
 ]
 
 
-def generate_ir() -> None:
+def generate_ir(methods=None) -> None:
     sys.path.insert(0, ROOT)
     from ccq_amd.gen import emit
-    emit.generate_all(os.path.join(HERE, "csrc"))
+    emit.generate_all(os.path.join(HERE, "csrc"), methods or emit.METHODS)
 
 
-def build(verbose: bool = False) -> str:
-    generate_ir()
+def build(verbose: bool = False, methods=None) -> str:
+    generate_ir(methods)
     cmd = [HIPCC] + FLAGS + SOURCES + ["-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)

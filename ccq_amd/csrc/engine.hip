@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -24,6 +25,11 @@
 #include "ir_ccsd_uhf.inc"
 #include "ir_ccsdt_rhf.inc"
 #include "ir_ccsdt_uhf.inc"
+#if __has_include("ir_ccsdtq_uhf.inc") && __has_include("ir_ccsdtq_rhf.inc")
+#include "ir_ccsdtq_rhf.inc"
+#include "ir_ccsdtq_uhf.inc"
+#define CCQ_HAVE_CCSDTQ 1
+#endif
 
 namespace {
 
@@ -84,9 +90,8 @@ struct Op {
   bool gemm = false;
   int64_t M = 1, N = 1, K = 1;
   int64_t t_rowA = 0, t_kA = 0, t_kB = 0, t_colB = 0, t_rowC = 0, t_colC = 0;
-  int a_kfast = 1, b_kfast = 1;
-  int bm = 64, bn = 64, nsplit = 1;
-  int64_t kchunk = 0;
+  ccqk::GemmArgs plan{};      // launch plan (kernel, tile, split-K, affine strides)
+  bool flat = false;          // add op over identical dense layouts
   double flops = 0, bytes = 0;
   std::string cls;           // profile class
 };
@@ -138,13 +143,13 @@ struct ccq_hip_ctx {
   double* vbb_ptr = nullptr;   // Vbb, or Vaa when ccq's v_bb equals v_aa
   std::map<std::string, std::unique_ptr<DevBuf>> resid_buf;
   std::vector<std::unique_ptr<DevBuf>> inter_buf;
-  int64_t* tables = nullptr;
-  int* tables32 = nullptr;   // GEMM offset tables
+  int* tables32 = nullptr;   // offset tables of every op
   int64_t* gather_tab = nullptr;
   std::vector<std::unique_ptr<DevBuf>> view_buf;
   struct Gather { int t; int64_t rows, cols, t_src_row, t_src_col, t_dst_row, t_dst_col; };
   std::vector<Gather> gathers;
   DevBuf ws;                 // split-K workspace
+  unsigned* counters = nullptr;   // split-K tile tickets (kept zero between launches)
   int* itab = nullptr;
   double* dtab = nullptr;
   // program
@@ -158,8 +163,10 @@ struct ccq_hip_ctx {
   std::vector<double> host_partial;
 
   ~ccq_hip_ctx() {
-    if (tables) (void)hipFree(tables);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (tables32) (void)hipFree(tables32);
+    if (counters) (void)hipFree(counters);
     if (gather_tab) (void)hipFree(gather_tab);
     if (itab) (void)hipFree(itab);
     if (dtab) (void)hipFree(dtab);
@@ -190,6 +197,13 @@ struct ccq_hip_ctx {
         {"t3aab", {oa, oa, ob, va, va, vb}},
         {"t3abb", {oa, ob, ob, va, vb, vb}},
         {"t3bbb", {ob, ob, ob, vb, vb, vb}},
+        // T4 is not part of ccq's t_vec (ccq keeps it in files,
+        // src/utils/cc_utils.f90:89-145); here it is resident after t_vec
+        {"t4aaaa", {oa, oa, oa, oa, va, va, va, va}},
+        {"t4aaab", {oa, oa, oa, ob, va, va, va, vb}},
+        {"t4aabb", {oa, oa, ob, ob, va, va, vb, vb}},
+        {"t4abbb", {oa, ob, ob, ob, va, vb, vb, vb}},
+        {"t4bbbb", {ob, ob, ob, ob, vb, vb, vb, vb}},
     };
     int64_t off = 0;
     for (auto& b : blocks) {
@@ -199,10 +213,10 @@ struct ccq_hip_ctx {
       blk_size[b.name] = sz;
       blk_dims[b.name] = b.d;
       off += sz;
+      if (std::string(b.name) == "t3bbb") t_size = off;
     }
-    t_size = off;
     vec_size = (level == 2) ? blk_off["t3aaa"] : t_size;
-    dev_t = (level == 2) ? blk_off["t3aaa"] : t_size;
+    dev_t = (level == 2) ? blk_off["t3aaa"] : (level == 3 ? t_size : off);
   }
 
   // -------------------------------------------------------------------------
@@ -412,9 +426,9 @@ struct ccq_hip_ctx {
   void gather_views() {
     for (auto& g : gathers) {
       const Tensor& t = tensors[g.t];
-      HIPCHK(ccqk::launch_perm_add(stream, g.rows, g.cols, t.src, gather_tab + g.t_src_row,
+      HIPCHK(ccqk::launch_gather64(stream, g.rows, g.cols, t.src, gather_tab + g.t_src_row,
                                    gather_tab + g.t_src_col, t.ptr, gather_tab + g.t_dst_row,
-                                   gather_tab + g.t_dst_col, 1.0, 0));
+                                   gather_tab + g.t_dst_col));
     }
     HIPCHK(hipStreamSynchronize(stream));
   }
@@ -451,25 +465,41 @@ struct ccq_hip_ctx {
   }
 
   int64_t ws_need = 0;
+  int64_t n_counters = 0;
+
+  // k * stride for every k (the offset table of a summed index run that is
+  // contiguous-by-stride in memory), or -1
+  static int64_t affine_stride(const std::vector<int64_t>& t) {
+    if (t.empty() || t[0] != 0) return -1;
+    if (t.size() == 1) return 0;
+    const int64_t s = t[1];
+    for (size_t k = 0; k < t.size(); ++k)
+      if (t[k] != (int64_t)k * s) return -1;
+    return s;
+  }
+
+  static std::string plan_tag(const ccqk::GemmArgs& g) {
+    std::string t;
+    if (g.kind == 1)
+      t = "r" + std::to_string(g.rm) + std::to_string(g.rn) + "w" + std::to_string(g.wk) +
+          (g.trans ? "t" : "");
+    else
+      t = std::to_string(g.bm) + "x" + std::to_string(g.bn) + "k" + std::to_string(g.bk);
+    return t + "s" + std::to_string(g.nsplit);
+  }
 
   void plan() {
-    std::vector<int64_t> host;
     std::vector<int> host32;
     auto push32 = [&](const std::vector<int64_t>& v) {
       int64_t o = (int64_t)host32.size();
       for (auto x : v) {
-        if (x < 0 || x > 0x7fffffffLL) throw Fail("tensor offset exceeds int32 GEMM tables");
+        if (x < 0 || x > 0x7fffffffLL) throw Fail("tensor offset exceeds int32 offset tables");
         host32.push_back((int)x);
       }
       return o;
     };
     std::vector<int> ih;
     std::vector<double> dh;
-    auto push = [&](const std::vector<int64_t>& v) {
-      int64_t o = (int64_t)host.size();
-      host.insert(host.end(), v.begin(), v.end());
-      return o;
-    };
     for (auto& sec : sections) {
       if (sec.kind == Section::Update) {
         sec.upd.dev_perm_off = (int)ih.size();
@@ -527,25 +557,21 @@ struct ccq_hip_ctx {
           op.t_colB = push32(cB);
           op.t_rowC = push32(rC);
           op.t_colC = push32(cC);
-          op.a_kfast = ks.empty() ? 1 : (stride_of(A, op.la, ks.back()) == 1 ? 1
+          ccqk::GemmArgs& g = op.plan;
+          g.M = op.M;
+          g.N = op.N;
+          g.K = op.K;
+          g.a_kfast = ks.empty() ? 1 : (stride_of(A, op.la, ks.back()) == 1 ? 1
                       : (!ms.empty() && stride_of(A, op.la, ms.back()) == 1 ? 0 : 1));
-          op.b_kfast = ks.empty() ? 1 : (stride_of(B, op.lb, ks.back()) == 1 ? 1
+          g.b_kfast = ks.empty() ? 1 : (stride_of(B, op.lb, ks.back()) == 1 ? 1
                       : (!ns.empty() && stride_of(B, op.lb, ns.back()) == 1 ? 0 : 1));
-          {
-            ccqk::GemmArgs g{};
-            g.M = op.M;
-            g.N = op.N;
-            g.K = op.K;
-            ccqk::gemm_plan(g, kWsCap);
-            op.bm = g.bm;
-            op.bn = g.bn;
-            op.nsplit = g.nsplit;
-            op.kchunk = g.kchunk;
-            if (g.nsplit > 1) ws_need = std::max<int64_t>(ws_need, (int64_t)g.nsplit * op.M * op.N);
-          }
+          g.kA_aff = affine_stride(kA);
+          g.kB_aff = affine_stride(kB);
+          ccqk::gemm_plan(g, kWsCap);
+          ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(g));
+          if (g.nsplit > 1) n_counters = std::max<int64_t>(n_counters, ccqk::gemm_tiles(g));
           op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +
-                   std::to_string(op.K) + "/" + std::to_string(op.bm) + "x" + std::to_string(op.bn) +
-                   "s" + std::to_string(op.nsplit);
+                   std::to_string(op.K) + "/" + plan_tag(g);
           op.flops = 2.0 * (double)op.M * (double)op.N * (double)(ks.empty() ? 1 : op.K);
           op.bytes = 8.0 * ((double)op.M * op.K + (double)op.K * op.N + 2.0 * op.M * op.N);
         } else {
@@ -556,20 +582,21 @@ struct ccq_hip_ctx {
           auto rC = offsets(C, op.lc, ms), cC = offsets(C, op.lc, ns);
           op.M = (int64_t)rA.size();
           op.N = (int64_t)cA.size();
-          op.t_rowA = push(rA);
-          op.t_kA = push(cA);
-          op.t_rowC = push(rC);
-          op.t_colC = push(cC);
+          // same layout on both sides: a flat (vectorised) axpy
+          op.flat = rA == rC && cA == cC && affine_stride(cA) == 1 && affine_stride(rA) == op.N;
+          op.t_rowA = push32(rA);
+          op.t_kA = push32(cA);
+          op.t_rowC = push32(rC);
+          op.t_colC = push32(cC);
           op.flops = (double)op.M * op.N;
           op.bytes = 8.0 * 3.0 * op.M * op.N;
-          op.cls = "add";
+          op.cls = op.flat ? "add:flat" : "add:perm";
         }
       }
     }
-    HIPCHK(hipMalloc(&tables, sizeof(int64_t) * std::max<size_t>(host.size(), 1)));
-    if (!host.empty())
-      HIPCHK(hipMemcpy(tables, host.data(), sizeof(int64_t) * host.size(), hipMemcpyHostToDevice));
     ws.alloc(std::max<int64_t>(ws_need, 1));
+    HIPCHK(hipMalloc(&counters, sizeof(unsigned) * std::max<int64_t>(n_counters, 1)));
+    HIPCHK(hipMemset(counters, 0, sizeof(unsigned) * std::max<int64_t>(n_counters, 1)));
     HIPCHK(hipMalloc(&tables32, sizeof(int) * std::max<size_t>(host32.size(), 1)));
     if (!host32.empty())
       HIPCHK(hipMemcpy(tables32, host32.data(), sizeof(int) * host32.size(), hipMemcpyHostToDevice));
@@ -589,10 +616,7 @@ struct ccq_hip_ctx {
     const int acc = op.assign ? 0 : 1;
     if (op.gemm) {
       const Tensor& B = tensors[op.b];
-      ccqk::GemmArgs g;
-      g.M = op.M;
-      g.N = op.N;
-      g.K = op.K;
+      ccqk::GemmArgs g = op.plan;
       g.A = A.ptr;
       g.rowA = tables32 + op.t_rowA;
       g.kA = tables32 + op.t_kA;
@@ -604,16 +628,18 @@ struct ccq_hip_ctx {
       g.colC = tables32 + op.t_colC;
       g.alpha = op.coef;
       g.accumulate = acc;
-      g.a_kfast = op.a_kfast;
-      g.b_kfast = op.b_kfast;
-      g.bm = op.bm;
-      g.bn = op.bn;
-      g.nsplit = op.nsplit;
-      g.kchunk = op.kchunk;
-      HIPCHK(ccqk::launch_gemm(stream, g, ws.p));
+      g.ws = ws.p;
+      g.counters = counters;
+      HIPCHK(ccqk::launch_gemm(stream, g));
     } else {
-      HIPCHK(ccqk::launch_perm_add(stream, op.M, op.N, A.ptr, tables + op.t_rowA, tables + op.t_kA,
-                                   C.ptr, tables + op.t_rowC, tables + op.t_colC, op.coef, acc));
+      const bool aligned =
+          ((reinterpret_cast<uintptr_t>(A.ptr) | reinterpret_cast<uintptr_t>(C.ptr)) & 15) == 0;
+      if (op.flat && aligned)
+        HIPCHK(ccqk::launch_axpy_flat(stream, op.M * op.N, A.ptr, C.ptr, op.coef, acc));
+      else
+        HIPCHK(ccqk::launch_perm_add(stream, op.M, op.N, A.ptr, tables32 + op.t_rowA,
+                                     tables32 + op.t_kA, C.ptr, tables32 + op.t_rowC,
+                                     tables32 + op.t_colC, op.coef, acc));
     }
   }
 
@@ -653,8 +679,35 @@ struct ccq_hip_ctx {
 
   void run_copy(const std::string& dst, const std::string& src) {
     if (blk_size.at(dst) != blk_size.at(src)) throw Fail("copy between blocks of different size");
-    HIPCHK(hipMemcpyAsync(tvec.p + blk_off.at(dst), tvec.p + blk_off.at(src),
-                          sizeof(double) * blk_size.at(src), hipMemcpyDeviceToDevice, stream));
+    HIPCHK(ccqk::launch_copy(stream, tvec.p + blk_off.at(dst), tvec.p + blk_off.at(src),
+                             blk_size.at(src)));
+  }
+
+  // A sweep is a fixed sequence of kernel launches on fixed buffers; it is
+  // captured once into a hipGraph and replayed (CCQ_HIP_GRAPH=0 disables).
+  bool use_graph = true;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+
+  void run_sweep() {
+    if (!use_graph || profile) {
+      sweep();
+      return;
+    }
+    if (!graph_exec) {
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        sweep();
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(stream, &graph));
+      HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(hipGraphLaunch(graph_exec, stream));
   }
 
   void sweep(int64_t nsec = -1) {
@@ -830,14 +883,15 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (ctx->oa < 0 || ctx->ob < 0 || ctx->va < 0 || ctx->vb < 0 || ctx->nc <= 0)
       throw Fail("inconsistent orbital counts");
     ctx->level = (int)opts->level;
-    if (ctx->level != 2 && ctx->level != 3)
-      throw Fail("this library implements CCSD (level 2) and CCSDT (level 3)");
+    if (ctx->level < 2 || ctx->level > 4)
+      throw Fail("level must be 2 (CCSD), 3 (CCSDT) or 4 (CCSDTQ)");
     ctx->rhf = opts->rhf != 0;
     if (ctx->rhf && ctx->oa != ctx->ob) throw Fail("rhf requires occ_a == occ_b");
     ctx->shift = opts->shift;
     ctx->diis = (int)opts->diis_space;
     if (ctx->diis < 1 || ctx->diis > ccqk::kMaxDiis) throw Fail("diis_space must be in 1..12");
     ctx->device = (int)opts->device;
+    if (const char* g = std::getenv("CCQ_HIP_GRAPH")) ctx->use_graph = std::string(g) != "0";
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->layout();
@@ -853,7 +907,14 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     ctx->ring.alloc((int64_t)(ctx->diis + 1) * ctx->vec_size);
     ctx->partial.alloc(kPartialBlocks * 128);
     if (ctx->level == 2) ctx->load_ir(ctx->rhf ? kIR_CCSD_RHF : kIR_CCSD_UHF);
-    else ctx->load_ir(ctx->rhf ? kIR_CCSDT_RHF : kIR_CCSDT_UHF);
+    else if (ctx->level == 3) ctx->load_ir(ctx->rhf ? kIR_CCSDT_RHF : kIR_CCSDT_UHF);
+    else {
+#ifdef CCQ_HAVE_CCSDTQ
+      ctx->load_ir(ctx->rhf ? kIR_CCSDTQ_RHF : kIR_CCSDTQ_UHF);
+#else
+      throw Fail("CCSDTQ program was not generated into this build");
+#endif
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
   });
   if (rc != 0) {
@@ -922,8 +983,8 @@ int64_t ccq_hip_t_size(const ccq_hip_ctx* ctx) { return ctx ? ctx->t_size : -1; 
 int ccq_hip_set_t(ccq_hip_ctx* ctx, const double* t_vec, int64_t t_size) {
   return guarded(ctx, [&] {
     if (t_size != ctx->t_size) throw Fail("t_size mismatch");
-    HIPCHK(hipMemcpyAsync(ctx->tvec.p, t_vec, sizeof(double) * ctx->dev_t, hipMemcpyHostToDevice,
-                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->tvec.p, t_vec, sizeof(double) * std::min(ctx->dev_t, ctx->t_size),
+                          hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
@@ -931,16 +992,38 @@ int ccq_hip_set_t(ccq_hip_ctx* ctx, const double* t_vec, int64_t t_size) {
 int ccq_hip_get_t(ccq_hip_ctx* ctx, double* t_vec, int64_t t_size) {
   return guarded(ctx, [&] {
     if (t_size != ctx->t_size) throw Fail("t_size mismatch");
-    HIPCHK(hipMemcpyAsync(t_vec, ctx->tvec.p, sizeof(double) * ctx->dev_t, hipMemcpyDeviceToHost,
-                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(t_vec, ctx->tvec.p, sizeof(double) * std::min(ctx->dev_t, ctx->t_size),
+                          hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     // blocks not resident (T3 for CCSD) are zero, as in ccq
     for (int64_t i = ctx->dev_t; i < ctx->t_size; ++i) t_vec[i] = 0.0;
   });
 }
 
+int64_t ccq_hip_t4_size(const ccq_hip_ctx* ctx) {
+  return ctx ? std::max<int64_t>(0, ctx->dev_t - ctx->t_size) : -1;
+}
+
+int ccq_hip_set_t4(ccq_hip_ctx* ctx, const double* t4, int64_t n) {
+  return guarded(ctx, [&] {
+    if (n != ctx->dev_t - ctx->t_size || n <= 0) throw Fail("t4 size mismatch");
+    HIPCHK(hipMemcpyAsync(ctx->tvec.p + ctx->t_size, t4, sizeof(double) * n, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ccq_hip_get_t4(ccq_hip_ctx* ctx, double* t4, int64_t n) {
+  return guarded(ctx, [&] {
+    if (n != ctx->dev_t - ctx->t_size || n <= 0) throw Fail("t4 size mismatch");
+    HIPCHK(hipMemcpyAsync(t4, ctx->tvec.p + ctx->t_size, sizeof(double) * n, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 int ccq_hip_update(ccq_hip_ctx* ctx) {
-  return guarded(ctx, [&] { ctx->sweep(); });
+  return guarded(ctx, [&] { ctx->run_sweep(); });
 }
 
 int ccq_hip_energy(ccq_hip_ctx* ctx, double* e) {

@@ -25,10 +25,19 @@ struct GemmArgs {
   int accumulate;
   int a_kfast;
   int b_kfast;
+  // kA[k] == k * kA_aff for every k when kA_aff >= 0 (same for B)
+  int64_t kA_aff = -1, kB_aff = -1;
   // launch plan (gemm_plan)
-  int bm = 64, bn = 64;
+  int kind = 0;            // 0: LDS-tiled kernel, 1: register-direct kernel
+  int trans = 0;           // run as C^T = B^T A^T (register-direct kernel)
+  int bm = 64, bn = 64, bk = 16;      // LDS-tiled tile
+  int rm = 1, rn = 1, wk = 1;         // register-direct wave tile / waves along K
   int nsplit = 1;
   int64_t kchunk = 0;
+  // split-K state: ws >= gemm_ws_need doubles, counters >= gemm_tiles
+  // zero-initialised unsigned ints (every launch leaves them zero)
+  double* ws = nullptr;
+  unsigned* counters = nullptr;
 };
 
 struct UpdateParams {
@@ -59,14 +68,26 @@ struct Coeffs {
   double c[kMaxDiis];
 };
 
-// Choose tile shape and split-K for a GEMM; ws_cap bounds nsplit*M*N.
+// Choose kernel, tile shape and split-K for a GEMM; ws_cap bounds the
+// split-K workspace (doubles).
 void gemm_plan(GemmArgs& g, int64_t ws_cap);
-// ws: split-K workspace of at least nsplit*M*N doubles (unused when nsplit == 1).
-hipError_t launch_gemm(hipStream_t s, const GemmArgs& g, double* ws);
-hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A,
-                           const int64_t* rowA, const int64_t* colA, double* C,
-                           const int64_t* rowC, const int64_t* colC, double alpha, int accumulate);
+// Output tiles of the planned launch, and split-K workspace it needs (0 when
+// nsplit == 1).
+int64_t gemm_tiles(const GemmArgs& g);
+int64_t gemm_ws_need(const GemmArgs& g);
+hipError_t launch_gemm(hipStream_t s, const GemmArgs& g);
+hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
+                           const int* colA, double* C, const int* rowC, const int* colC,
+                           double alpha, int accumulate);
+// Y[rowY[m] + colY[n]] = X[rowX[m] + colX[n]] (int64 tables; view set-up)
+hipError_t launch_gather64(hipStream_t s, int64_t M, int64_t N, const double* X,
+                           const int64_t* rowX, const int64_t* colX, double* Y,
+                           const int64_t* rowY, const int64_t* colY);
+// C (+)= alpha A for identically laid out dense arrays of n doubles (16-byte aligned)
+hipError_t launch_axpy_flat(hipStream_t s, int64_t n, const double* A, double* C, double alpha,
+                            int accumulate);
 hipError_t launch_fill(hipStream_t s, double* x, int64_t n, double v);
+hipError_t launch_copy(hipStream_t s, double* y, const double* x, int64_t n);
 hipError_t launch_update(hipStream_t s, const UpdateParams& P, double* t, const double* U,
                          const int* perms, const double* signs, const double* fa,
                          const double* fb);

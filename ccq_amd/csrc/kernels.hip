@@ -25,13 +25,102 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 namespace ccqk {
 
-constexpr int BK = 16;
+// ---------------------------------------------------------------------------
+// GEMM building blocks
+
+// Offset of summed index k in an operand: an int32 table, or k * aff when the
+// table is affine (aff >= 0), which saves the dependent table load.
+__device__ __forceinline__ int64_t koff(const int* __restrict__ tab, int64_t aff, int64_t k) {
+  return aff >= 0 ? k * aff : (int64_t)tab[k];
+}
+
+__device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n, double s) {
+  double* p = g.C + (int64_t)g.rowC[m] + g.colC[n];
+  const double v = g.alpha * s;
+  if (g.accumulate) *p += v; else *p = v;
+}
+
+// sum_{i < cnt} q[i * stride] with four interleaved partial sums (fixed order)
+__device__ __forceinline__ double sum_strided(const double* __restrict__ q, int64_t stride, int cnt) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = 0;
+  for (; i + 4 <= cnt; i += 4) {
+    a0 += q[0];
+    a1 += q[stride];
+    a2 += q[2 * stride];
+    a3 += q[3 * stride];
+    q += 4 * stride;
+  }
+  for (; i < cnt; ++i) {
+    a0 += q[0];
+    q += stride;
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+
+// Split-K completion, shared by both GEMM kernels.  Each of the nsplit
+// workgroups of an output tile has stored its partial tile (TBM x TBN,
+// row-major) at ws + (tile * nsplit + z) * TBM * TBN and calls splitk_last:
+// it publishes the partial (release fence) and takes a ticket on the tile's
+// counter.  The workgroup holding the last ticket resets the counter for the
+// next launch, acquires, and runs splitk_finish, which sums the partials in an
+// order fixed by (nsplit, tile extent) -- the result does not depend on which
+// workgroup arrived last -- and writes C.  No workgroup ever waits on another.
+constexpr int kRed = 2048;   // LDS doubles splitk_finish may use
+
+__device__ __forceinline__ bool splitk_last(const GemmArgs& g, int tile) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned prev = atomicAdd(g.counters + tile, 1u);
+    last = prev == (unsigned)(g.nsplit - 1) ? 1 : 0;
+    if (last) g.counters[tile] = 0u;
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  return true;
+}
+
+template <int TBM, int TBN>
+__device__ void splitk_finish(const GemmArgs& g, const double* __restrict__ part, int64_t m0,
+                              int64_t n0, double* red) {
+  constexpr int TE = TBM * TBN;
+  const int mv = (int)(g.M - m0 < TBM ? g.M - m0 : TBM);
+  const int nv = (int)(g.N - n0 < TBN ? g.N - n0 : TBN);
+  const int E = mv * nv;
+  const int ns = g.nsplit;
+  // P threads per output element when the tile has few valid elements
+  int P = 1;
+  while (2 * P <= ns && E * 2 * P <= kRed) P *= 2;
+  const int tid = threadIdx.x;
+  if (P == 1) {
+    for (int e = tid; e < E; e += 256) {
+      const int r = e / nv, c = e - r * nv;
+      store_c(g, m0 + r, n0 + c, sum_strided(part + r * TBN + c, (int64_t)TE, ns));
+    }
+    return;
+  }
+  for (int w = tid; w < E * P; w += 256) {
+    const int e = w / P, p = w - e * P;
+    const int r = e / nv, c = e - r * nv;
+    red[w] = sum_strided(part + (int64_t)p * TE + r * TBN + c, (int64_t)P * TE, (ns - p + P - 1) / P);
+  }
+  __syncthreads();
+  for (int e = tid; e < E; e += 256) {
+    double s = 0.0;
+    for (int p = 0; p < P; ++p) s += red[e * P + p];
+    const int r = e / nv, c = e - r * nv;
+    store_c(g, m0 + r, n0 + c, s);
+  }
+}
 
 // LDS tile of an operand: R x BK values.  KF (k-fastest in global memory):
 // stored row-major [R][BK+1] so a 16-lane group writing consecutive k and
 // the MFMA fragment read (16 consecutive rows, one k) are both bank-conflict
 // free.  Otherwise stored k-major [BK][R+2].
-template <int R, bool KF>
+template <int R, bool KF, int BK>
 struct LdsTile {
   static constexpr int size = KF ? R * (BK + 1) : BK * (R + 2);
   __device__ static int at(int row, int k) { return KF ? row * (BK + 1) + k : k * (R + 2) + row; }
@@ -40,7 +129,7 @@ struct LdsTile {
 // Per-thread loader of one operand slab (R rows x BK k-values per slab).
 // KF: thread owns k = tid % BK and rows tid / BK + j * (256 / BK);
 // else: thread owns row tid % R and k = tid / R + j * (256 / R).
-template <int R, bool KF, bool ISB>
+template <int R, bool KF, int BK>
 struct Loader {
   static constexpr int L = R * BK / 256;
   int row_off[L];        // table offset of each owned row (or -1)
@@ -59,57 +148,59 @@ struct Loader {
     }
   }
   __device__ void fetch(double (&reg)[L], const double* __restrict__ X, const int* __restrict__ ktab,
-                        int64_t k0, int64_t kend) const {
+                        int64_t kaff, int64_t k0, int64_t kend) const {
     if (KF) {
       const int64_t gk = k0 + kk[0];
       const bool kin = gk < kend;
-      const int ko = kin ? ktab[gk] : 0;
+      const int64_t ko = kin ? koff(ktab, kaff, gk) : 0;
 #pragma unroll
       for (int j = 0; j < L; ++j) reg[j] = (kin && row_off[j] >= 0) ? X[(int64_t)row_off[j] + ko] : 0.0;
     } else {
 #pragma unroll
       for (int j = 0; j < L; ++j) {
         const int64_t gk = k0 + kk[j];
-        reg[j] = (gk < kend && row_off[j] >= 0) ? X[(int64_t)row_off[j] + ktab[gk]] : 0.0;
+        reg[j] = (gk < kend && row_off[j] >= 0) ? X[(int64_t)row_off[j] + koff(ktab, kaff, gk)] : 0.0;
       }
     }
   }
   __device__ void stash(double* lds, const double (&reg)[L]) const {
 #pragma unroll
-    for (int j = 0; j < L; ++j) lds[LdsTile<R, KF>::at(rr[j], kk[j])] = reg[j];
+    for (int j = 0; j < L; ++j) lds[LdsTile<R, KF, BK>::at(rr[j], kk[j])] = reg[j];
   }
 };
 
-// Tile GEMM C(M x N) (+)= alpha * A(M x K) B(K x N) on v_mfma_f64_16x16x4f64.
-// 256 threads = 4 waves in a 2 x 2 grid; each wave owns a (BM/2) x (BN/2)
-// sub-tile of 16x16 MFMA tiles.  Operands are gathered through int32 offset
-// tables (rowA/kA, kB/colB); the next K slab is fetched into registers while
-// the current one is multiplied out of LDS.
-// Split-K: blockIdx.z owns K range [z*kchunk, (z+1)*kchunk); with nsplit > 1
-// the tile goes to ws[z][m][n] and gemm_splitk_reduce finishes it.
-template <int BM, int BN, bool AKF, bool BKF>
-__global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g, int64_t kchunk, double* __restrict__ ws) {
+// LDS-tiled GEMM C(M x N) (+)= alpha * A(M x K) B(K x N) on
+// v_mfma_f64_16x16x4f64, for products with both output extents large
+// (MFMA bound).  256 threads = 4 waves in a 2 x 2 grid; each wave owns a
+// (BM/2) x (BN/2) sub-tile of 16x16 MFMA tiles.  Operands are gathered through
+// int32 offset tables (rowA/kA, kB/colB); the next K slab is fetched into
+// registers while the current one is multiplied out of LDS.  blockIdx.z owns
+// K range [z*kchunk, (z+1)*kchunk) (split-K, completed by splitk_finish).
+template <int BM, int BN, int BK, bool AKF, bool BKF>
+__global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
-  using TA = LdsTile<BM, AKF>;
-  using TB = LdsTile<BN, BKF>;
-  __shared__ double As[TA::size];
-  __shared__ double Bs[TB::size];
+  using TA = LdsTile<BM, AKF, BK>;
+  using TB = LdsTile<BN, BKF, BK>;
+  constexpr int SMEM = TA::size + TB::size > kRed ? TA::size + TB::size : kRed;
+  __shared__ double smem[SMEM];
+  double* As = smem;
+  double* Bs = smem + TA::size;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int64_t m0 = (int64_t)blockIdx.x * BM;
   const int64_t n0 = (int64_t)blockIdx.y * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
-  const int64_t kend = (kbeg + kchunk < g.K) ? kbeg + kchunk : g.K;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  const int64_t kend = (kbeg + g.kchunk < g.K) ? kbeg + g.kchunk : g.K;
   const int wm = (wave & 1) * WM;
   const int wn = (wave >> 1) * WN;
 
-  Loader<BM, AKF, false> la;
-  Loader<BN, BKF, true> lb;
+  Loader<BM, AKF, BK> la;
+  Loader<BN, BKF, BK> lb;
   la.init(g.rowA, m0, g.M, tid);
   lb.init(g.colB, n0, g.N, tid);
-  double ra[Loader<BM, AKF, false>::L], rb[Loader<BN, BKF, true>::L];
+  double ra[Loader<BM, AKF, BK>::L], rb[Loader<BN, BKF, BK>::L];
 
   d4 acc[TM][TN];
 #pragma unroll
@@ -119,8 +210,8 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g, int64_t kchun
 
   const int64_t nsteps = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   if (nsteps > 0) {
-    la.fetch(ra, g.A, g.kA, kbeg, kend);
-    lb.fetch(rb, g.B, g.kB, kbeg, kend);
+    la.fetch(ra, g.A, g.kA, g.kA_aff, kbeg, kend);
+    lb.fetch(rb, g.B, g.kB, g.kB_aff, kbeg, kend);
   }
   for (int64_t s = 0; s < nsteps; ++s) {
     __syncthreads();
@@ -128,8 +219,8 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g, int64_t kchun
     lb.stash(Bs, rb);
     __syncthreads();
     if (s + 1 < nsteps) {
-      la.fetch(ra, g.A, g.kA, kbeg + (s + 1) * BK, kend);
-      lb.fetch(rb, g.B, g.kB, kbeg + (s + 1) * BK, kend);
+      la.fetch(ra, g.A, g.kA, g.kA_aff, kbeg + (s + 1) * BK, kend);
+      lb.fetch(rb, g.B, g.kB, g.kB_aff, kbeg + (s + 1) * BK, kend);
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
@@ -150,56 +241,246 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g, int64_t kchun
 
   // D layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l, register v
   // holds row (l/16) + 4*v, column l%16
-  const int64_t M = g.M, N = g.N;
+  if (g.nsplit == 1) {
 #pragma unroll
-  for (int j = 0; j < TN; ++j) {
-    const int64_t gn = n0 + wn + 16 * j + (lane & 15);
-    if (gn >= N) continue;
-    const int64_t cc = ws ? gn : (int64_t)g.colC[gn];
+    for (int j = 0; j < TN; ++j) {
+      const int64_t gn = n0 + wn + 16 * j + (lane & 15);
+      if (gn >= g.N) continue;
 #pragma unroll
-    for (int i = 0; i < TM; ++i) {
+      for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int64_t gm = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
-        if (gm >= M) continue;
-        if (ws) {
-          ws[((int64_t)blockIdx.z * M + gm) * N + cc] = acc[i][j][v];
-        } else {
-          double* p = g.C + (int64_t)g.rowC[gm] + cc;
-          const double val = g.alpha * acc[i][j][v];
-          if (g.accumulate) *p += val; else *p = val;
+        for (int v = 0; v < 4; ++v) {
+          const int64_t gm = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
+          if (gm < g.M) store_c(g, gm, gn, acc[i][j][v]);
         }
+    }
+    return;
+  }
+  const int tile = blockIdx.x + gridDim.x * blockIdx.y;
+  double* part = g.ws + ((int64_t)tile * g.nsplit + blockIdx.z) * (BM * BN);
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        part[(wm + 16 * i + (lane >> 4) + 4 * v) * BN + wn + 16 * j + (lane & 15)] = acc[i][j][v];
+  if (!splitk_last(g, tile)) return;
+  splitk_finish<BM, BN>(g, g.ws + (int64_t)tile * g.nsplit * (BM * BN), m0, n0, smem);
+}
+
+// Register-direct GEMM for skinny and dot-shaped products (an output extent
+// of at most a few MFMA tiles; HBM / latency bound).  No LDS staging: in a
+// 16-deep k step, lane l loads row l%16 of each 16-row block at
+// k = kb + 4*(l/16) + s, s = 0..3, and MFMA s contracts exactly those k
+// (the A/B fragment layout of v_mfma_f64_16x16x4f64 with the 4 MFMAs of the
+// step taking k in the order kb + 4q + s, q = 0..3).  The next step's values
+// are loaded while the current step is multiplied.
+// Workgroup = 4 waves: WK along K (same output tile, interleaved k steps,
+// combined through LDS in wave order), 4/WK along M.  Wave tile (16 RM) x (16 RN).
+template <int RM, int RN, int WK>
+struct RegStep {
+  double a[RM][4];
+  double b[RN][4];
+};
+
+template <int RM, int RN, int WK>
+__device__ __forceinline__ void reg_load(RegStep<RM, RN, WK>& st, const GemmArgs& g, const int (&ra)[RM],
+                                         const int (&cb)[RN], int q, int64_t kb, int64_t kend) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int64_t gk = kb + 4 * q + s;
+    const bool in = gk < kend;
+    const int64_t oa = in ? koff(g.kA, g.kA_aff, gk) : 0;
+    const int64_t ob = in ? koff(g.kB, g.kB_aff, gk) : 0;
+#pragma unroll
+    for (int i = 0; i < RM; ++i) st.a[i][s] = (in && ra[i] >= 0) ? g.A[(int64_t)ra[i] + oa] : 0.0;
+#pragma unroll
+    for (int j = 0; j < RN; ++j) st.b[j][s] = (in && cb[j] >= 0) ? g.B[ob + cb[j]] : 0.0;
+  }
+}
+
+template <int RM, int RN, int WK>
+__device__ __forceinline__ void reg_mma(d4 (&acc)[RM][RN], const RegStep<RM, RN, WK>& st) {
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int i = 0; i < RM; ++i)
+#pragma unroll
+      for (int j = 0; j < RN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(st.a[i][s], st.b[j][s], acc[i][j], 0, 0, 0);
+}
+
+template <int RM, int RN, int WK>
+__global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
+  constexpr int WMW = 4 / WK;
+  constexpr int TBM = WMW * 16 * RM, TBN = 16 * RN;
+  constexpr int WACC = 256 * RM * RN;   // doubles of one wave's accumulators
+  constexpr int SMEM = (WK - 1) * WMW * WACC > kRed ? (WK - 1) * WMW * WACC : kRed;
+  __shared__ double red[SMEM];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wk = wave % WK, wmi = wave / WK;
+  const int r = lane & 15, q = lane >> 4;
+  const int64_t m0 = (int64_t)blockIdx.x * TBM, n0 = (int64_t)blockIdx.y * TBN;
+  const int64_t wm0 = m0 + wmi * 16 * RM;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  const int64_t kend = (kbeg + g.kchunk < g.K) ? kbeg + g.kchunk : g.K;
+  int ra[RM], cb[RN];
+#pragma unroll
+  for (int i = 0; i < RM; ++i) {
+    const int64_t m = wm0 + 16 * i + r;
+    ra[i] = m < g.M ? g.rowA[m] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < RN; ++j) {
+    const int64_t n = n0 + 16 * j + r;
+    cb[j] = n < g.N ? g.colB[n] : -1;
+  }
+  d4 acc[RM][RN];
+#pragma unroll
+  for (int i = 0; i < RM; ++i)
+#pragma unroll
+    for (int j = 0; j < RN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+
+  const int64_t nsteps = kend > kbeg ? (kend - kbeg + 15) / 16 : 0;
+  RegStep<RM, RN, WK> s0, s1;
+  int64_t st = wk;
+  if (st < nsteps) reg_load(s0, g, ra, cb, q, kbeg + 16 * st, kend);
+  while (st < nsteps) {
+    const int64_t n1 = st + WK;
+    if (n1 < nsteps) reg_load(s1, g, ra, cb, q, kbeg + 16 * n1, kend);
+    reg_mma(acc, s0);
+    if (n1 >= nsteps) break;
+    const int64_t n2 = n1 + WK;
+    if (n2 < nsteps) reg_load(s0, g, ra, cb, q, kbeg + 16 * n2, kend);
+    reg_mma(acc, s1);
+    st = n2;
+  }
+
+  if (WK > 1) {
+    // waves wk > 0 hand their accumulators to wave wk == 0 of the same M slot
+    if (wk > 0) {
+      double* dst = red + ((wk - 1) * WMW + wmi) * WACC;
+#pragma unroll
+      for (int i = 0; i < RM; ++i)
+#pragma unroll
+        for (int j = 0; j < RN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) dst[((i * RN + j) * 4 + v) * 64 + lane] = acc[i][j][v];
+    }
+    __syncthreads();
+    if (wk == 0) {
+      for (int w = 1; w < WK; ++w) {
+        const double* src = red + ((w - 1) * WMW + wmi) * WACC;
+#pragma unroll
+        for (int i = 0; i < RM; ++i)
+#pragma unroll
+          for (int j = 0; j < RN; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[i][j][v] += src[((i * RN + j) * 4 + v) * 64 + lane];
       }
+    }
+  }
+
+  if (g.nsplit == 1) {
+    if (wk != 0) return;
+#pragma unroll
+    for (int j = 0; j < RN; ++j) {
+      const int64_t gn = n0 + 16 * j + r;
+      if (gn >= g.N) continue;
+#pragma unroll
+      for (int i = 0; i < RM; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t gm = wm0 + 16 * i + q + 4 * v;
+          if (gm < g.M) store_c(g, gm, gn, acc[i][j][v]);
+        }
+    }
+    return;
+  }
+  const int tile = blockIdx.x + gridDim.x * blockIdx.y;
+  if (wk == 0) {
+    double* part = g.ws + ((int64_t)tile * g.nsplit + blockIdx.z) * (TBM * TBN);
+#pragma unroll
+    for (int j = 0; j < RN; ++j)
+#pragma unroll
+      for (int i = 0; i < RM; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          part[(wmi * 16 * RM + 16 * i + q + 4 * v) * TBN + 16 * j + r] = acc[i][j][v];
+  }
+  if (!splitk_last(g, tile)) return;
+  splitk_finish<TBM, TBN>(g, g.ws + (int64_t)tile * g.nsplit * (TBM * TBN), m0, n0, red);
+}
+
+// C(perm) (+)= alpha A through int32 offset tables: element (m, n) reads
+// A[rowA[m] + colA[n]] and updates C[rowC[m] + colC[n]].  A block covers
+// 256 >> tn_shift rows x (1 << tn_shift) columns per pass.
+__global__ __launch_bounds__(256) void perm_add(int64_t M, int N, int tn_shift,
+                                                const double* __restrict__ A,
+                                                const int* __restrict__ rowA,
+                                                const int* __restrict__ colA,
+                                                double* __restrict__ C,
+                                                const int* __restrict__ rowC,
+                                                const int* __restrict__ colC, double alpha,
+                                                int accumulate) {
+  const int tn = 1 << tn_shift;
+  const int tm = 256 >> tn_shift;
+  const int n0 = threadIdx.x & (tn - 1);
+  for (int64_t m = (int64_t)blockIdx.x * tm + (threadIdx.x >> tn_shift); m < M;
+       m += (int64_t)gridDim.x * tm) {
+    const int64_t ra = rowA[m], rc = rowC[m];
+    for (int n = n0; n < N; n += tn) {
+      const double v = alpha * A[ra + colA[n]];
+      double* p = C + rc + colC[n];
+      if (accumulate) *p += v; else *p = v;
     }
   }
 }
 
-// C = beta C + alpha * sum_z ws[z] in fixed z order (deterministic).
-__global__ void gemm_splitk_reduce(GemmArgs g, int nsplit, const double* __restrict__ ws) {
-  const int64_t total = g.M * g.N;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+// Y[rowY[m] + colY[n]] = X[rowX[m] + colX[n]] with int64 tables (copies a
+// view out of a base integral array that may exceed 2^31 elements)
+__global__ __launch_bounds__(256) void gather64(int64_t M, int64_t N, const double* __restrict__ X,
+                                                const int64_t* __restrict__ rowX,
+                                                const int64_t* __restrict__ colX,
+                                                double* __restrict__ Y,
+                                                const int64_t* __restrict__ rowY,
+                                                const int64_t* __restrict__ colY) {
+  for (int64_t m = blockIdx.x; m < M; m += gridDim.x)
+    for (int64_t n = threadIdx.x; n < N; n += blockDim.x) Y[rowY[m] + colY[n]] = X[rowX[m] + colX[n]];
+}
+
+// C (+)= alpha A over identically laid out dense tensors, two doubles per thread
+__global__ __launch_bounds__(256) void axpy_flat(int64_t n, const double* __restrict__ A,
+                                                 double* __restrict__ C, double alpha,
+                                                 int accumulate) {
+  const int64_t n2 = n / 2;
+  const double2* A2 = reinterpret_cast<const double2*>(A);
+  double2* C2 = reinterpret_cast<double2*>(C);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n2;
        e += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int z = 0; z < nsplit; ++z) s += ws[(int64_t)z * total + e];
-    const int64_t m = e / g.N, n = e - m * g.N;
-    double* p = g.C + (int64_t)g.rowC[m] + g.colC[n];
-    const double val = g.alpha * s;
-    if (g.accumulate) *p += val; else *p = val;
+    const double2 a = A2[e];
+    double2 c;
+    if (accumulate) {
+      c = C2[e];
+      c.x += alpha * a.x;
+      c.y += alpha * a.y;
+    } else {
+      c.x = alpha * a.x;
+      c.y = alpha * a.y;
+    }
+    C2[e] = c;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && (n & 1)) {
+    const double v = alpha * A[n - 1];
+    if (accumulate) C[n - 1] += v; else C[n - 1] = v;
   }
 }
 
-__global__ void perm_add(int64_t M, int64_t N, const double* __restrict__ A,
-                         const int64_t* __restrict__ rowA, const int64_t* __restrict__ colA,
-                         double* __restrict__ C, const int64_t* __restrict__ rowC,
-                         const int64_t* __restrict__ colC, double alpha, int accumulate) {
-  const int64_t total = M * N;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m = e / N, n = e - m * N;
-    const double v = alpha * A[rowA[m] + colA[n]];
-    double* p = C + rowC[m] + colC[n];
-    if (accumulate) *p += v; else *p = v;
-  }
+__global__ void copy_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = x[e];
 }
 
 __global__ void fill(double* __restrict__ x, int64_t n, double v) {
@@ -377,76 +658,236 @@ static int grid_for(int64_t n, int threads, int cap = 4096) {
   return (int)g;
 }
 
+// Register-direct kernel variants: (RM, RN, WK).  Output tile
+// ((4/WK) * 16 * RM) x (16 * RN).
+struct RegShape { int rm, rn, wk; };
+static const RegShape kRegShapes[] = {{4, 1, 1}, {2, 2, 1}, {1, 1, 4}, {2, 1, 4},
+                                      {4, 1, 4}, {6, 1, 4}, {2, 2, 4}};
+
+static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+static void tile_dims(const GemmArgs& g, int64_t& tbm, int64_t& tbn) {
+  if (g.kind == 1) {
+    tbm = (4 / g.wk) * 16 * g.rm;
+    tbn = 16 * g.rn;
+  } else {
+    tbm = g.bm;
+    tbn = g.bn;
+  }
+}
+
+int64_t gemm_tiles(const GemmArgs& g) {
+  int64_t tbm, tbn;
+  tile_dims(g, tbm, tbn);
+  const int64_t M = g.trans ? g.N : g.M, N = g.trans ? g.M : g.N;
+  return cdiv(M, tbm) * cdiv(N, tbn);
+}
+
+int64_t gemm_ws_need(const GemmArgs& g) {
+  if (g.nsplit <= 1) return 0;
+  int64_t tbm, tbn;
+  tile_dims(g, tbm, tbn);
+  return gemm_tiles(g) * g.nsplit * tbm * tbn;
+}
+
+// Split-K completion: every workgroup stores its partial tile; the last one
+// of a tile reads all nsplit partials (~64 B/cycle into one CU, a few
+// latency-bound rounds) and pays a fence/atomic round trip.
+static double splitk_cycles(int64_t tiles, int64_t split, int64_t tile_elems, int64_t valid_elems) {
+  if (split <= 1) return 0.0;
+  const double hbm = 8.0e12 / 2.4e9;
+  const double write = (double)tiles * split * tile_elems * 8.0 / hbm;
+  int64_t P = 1;
+  while (2 * P <= split && valid_elems * 2 * P <= 2048) P *= 2;
+  const double per_thread = (double)cdiv(valid_elems * P, 256) * cdiv(split, P);
+  const double read = (double)valid_elems * split * 8.0 / 64.0;
+  return write + std::max(read, per_thread / 4.0 * 700.0) + 4000.0;
+}
+
 void gemm_plan(GemmArgs& g, int64_t ws_cap) {
-  // Cost model in MFMA cycles of the busiest SIMD.  A 16x16x4 FP64 MFMA
-  // occupies a SIMD for 64 cycles (78.6 TF/s over 256 CUs x 4 SIMDs at
-  // 2.4 GHz); a WG of 4 waves keeps each SIMD busy for (BM/32)(BN/32)*4 MFMAs
-  // per 16-deep K slab.  Resident WGs per CU (from VGPR/LDS use): 2 for
-  // 128x128, 3 for 128x64 / 64x128, 4 for 64x64.  Split-K adds the
-  // workspace round trip and a reduction pass.
+  // Cost model in cycles (2.4 GHz).  A 16x16x4 FP64 MFMA occupies a SIMD for
+  // 64 cycles (78.6 TF/s over 256 CUs x 4 SIMDs).
+  const double cus = 256.0;
+  const double hbm = 8.0e12 / 2.4e9;   // bytes per cycle
+  double best = 1e300;
+  auto take = [&](double cyc, int kind, int trans, int bm, int bn, int bk, const RegShape* r,
+                  int64_t split, int64_t kchunk) {
+    if (cyc >= best) return;
+    best = cyc;
+    g.kind = kind;
+    g.trans = trans;
+    g.bm = bm;
+    g.bn = bn;
+    g.bk = bk;
+    g.rm = r ? r->rm : 1;
+    g.rn = r ? r->rn : 1;
+    g.wk = r ? r->wk : 1;
+    g.nsplit = (int)split;
+    g.kchunk = kchunk;
+  };
+
+  // LDS-tiled kernel.  A WG of 4 waves keeps each SIMD busy for
+  // (BM/32)(BN/32)*BK/4 MFMAs per K slab; resident WGs per CU (VGPR/LDS):
+  // 2 for 128x128, 3 for 128x64 / 64x128, 4 for 64x64.  A CU holding a single
+  // WG cannot hide the global-load latency of a slab (~1500 cycles); every
+  // slab costs two barriers and the LDS round trip (~300 cycles).
   struct Shape { int bm, bn, occ; };
   const Shape shapes[4] = {{128, 128, 2}, {128, 64, 3}, {64, 128, 3}, {64, 64, 4}};
-  const double cus = 256.0;
-  const int64_t ksteps = (g.K + BK - 1) / BK;
-  double best = 1e300;
-  for (const Shape& sh : shapes) {
-    const int64_t tiles = ((g.M + sh.bm - 1) / sh.bm) * ((g.N + sh.bn - 1) / sh.bn);
-    const double slab = (sh.bm / 32.0) * (sh.bn / 32.0) * 4.0 * 64.0;   // cycles per K slab
-    const int64_t maxsplit = std::max<int64_t>(1, std::min<int64_t>(ksteps / 2, 4096));
-    for (int64_t split = 1; split <= maxsplit; split *= 2) {
-      if (split > 1 && split * g.M * g.N > ws_cap) break;
-      const int64_t steps = (ksteps + split - 1) / split;
-      // MFMA work is balanced over CUs in whole WGs; a CU holding a single WG
-      // cannot hide the global-load latency of a slab (~1500 cycles).
-      const double per_cu = std::ceil((double)tiles * split / cus);
-      const double resident = std::min<double>(per_cu, sh.occ);
-      double cyc = per_cu * (steps + 3) * slab;
-      if (resident < 2) cyc += per_cu * steps * 1500.0;
-      if (split > 1) cyc += (2.0 * split * g.M * g.N * 8.0) / 8.0e12 * 2.4e9 + 12000.0;
-      if (cyc < best) {
-        best = cyc;
-        g.bm = sh.bm;
-        g.bn = sh.bn;
-        g.nsplit = (int)split;
-        g.kchunk = steps * BK;
+  for (int bk : {16, 32}) {
+    const int64_t ksteps = cdiv(g.K, bk);
+    for (const Shape& sh : shapes) {
+      if (bk == 32 && sh.bm == 128 && sh.bn == 128) continue;   // exceeds 256 VGPRs
+      const int64_t tiles = cdiv(g.M, sh.bm) * cdiv(g.N, sh.bn);
+      const int64_t valid = std::min<int64_t>(g.M, sh.bm) * std::min<int64_t>(g.N, sh.bn);
+      const double slab = (sh.bm / 32.0) * (sh.bn / 32.0) * (bk / 4.0) * 64.0;
+      const int64_t maxsplit = std::max<int64_t>(1, std::min<int64_t>(ksteps / 2, 512));
+      for (int64_t split = 1; split <= maxsplit; split *= 2) {
+        if (split > 1 && tiles * split * sh.bm * sh.bn > ws_cap) break;
+        const int64_t steps = cdiv(ksteps, split);
+        const double per_cu = std::ceil((double)tiles * split / cus);
+        const double resident = std::min<double>(per_cu, sh.occ);
+        double cyc = per_cu * (steps + 2) * (slab + 300.0);
+        if (resident < 2) cyc += per_cu * steps * 1500.0;
+        cyc += splitk_cycles(tiles, split, (int64_t)sh.bm * sh.bn, valid);
+        take(cyc, 0, 0, sh.bm, sh.bn, bk, nullptr, split, steps * bk);
+      }
+    }
+  }
+
+  // Register-direct kernel for products with a skinny output side (N, or M
+  // when run transposed).  Per 16-deep k step a wave issues 4*RM*RN MFMAs and
+  // waits on one load round trip at most partially hidden by the prefetch;
+  // A is re-read once per tile column and B once per tile row.
+  for (int trans = 0; trans < 2; ++trans) {
+    const int64_t M = trans ? g.N : g.M, N = trans ? g.M : g.N;
+    for (const RegShape& r : kRegShapes) {
+      const int64_t tbm = (4 / r.wk) * 16 * r.rm, tbn = 16 * r.rn;
+      if (N > 2 * tbn) continue;
+      const int64_t tiles = cdiv(M, tbm) * cdiv(N, tbn);
+      const int64_t valid = std::min<int64_t>(M, tbm) * std::min<int64_t>(N, tbn);
+      const int64_t k16 = cdiv(g.K, 16);
+      const double step = std::max(4.0 * r.rm * r.rn * 64.0, 700.0);
+      const double bytes = 8.0 * ((double)M * g.K * cdiv(N, tbn) + (double)g.K * N * cdiv(M, tbm));
+      for (int64_t split = 1; split <= std::min<int64_t>(k16, 1024); split *= 2) {
+        if (split > 1 && tiles * split * tbm * tbn > ws_cap) break;
+        const int64_t steps = cdiv(k16, split);
+        const double rounds = std::ceil((double)tiles * split / (cus * 4.0));
+        double cyc = rounds * ((double)cdiv(steps, r.wk) * step + 1500.0);
+        cyc = std::max(cyc, bytes / hbm);
+        cyc += splitk_cycles(tiles, split, tbm * tbn, valid);
+        take(cyc, 1, trans, 64, 64, 16, &r, split, steps * 16);
       }
     }
   }
 }
 
-template <int BM, int BN>
-static void launch_tile(hipStream_t s, dim3 grid, const GemmArgs& g, double* w) {
+template <int BM, int BN, int KS>
+static void launch_tile(hipStream_t s, dim3 grid, const GemmArgs& g) {
   if (g.a_kfast && g.b_kfast)
-    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, true, true>), grid, dim3(256), 0, s, g, g.kchunk, w);
+    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, KS, true, true>), grid, dim3(256), 0, s, g);
   else if (g.a_kfast)
-    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, true, false>), grid, dim3(256), 0, s, g, g.kchunk, w);
+    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, KS, true, false>), grid, dim3(256), 0, s, g);
   else if (g.b_kfast)
-    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, false, true>), grid, dim3(256), 0, s, g, g.kchunk, w);
+    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, KS, false, true>), grid, dim3(256), 0, s, g);
   else
-    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, false, false>), grid, dim3(256), 0, s, g, g.kchunk, w);
+    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, KS, false, false>), grid, dim3(256), 0, s, g);
 }
 
-hipError_t launch_gemm(hipStream_t s, const GemmArgs& g, double* ws) {
-  dim3 grid((unsigned)((g.M + g.bm - 1) / g.bm), (unsigned)((g.N + g.bn - 1) / g.bn),
-            (unsigned)g.nsplit);
-  double* w = g.nsplit > 1 ? ws : nullptr;
-  if (g.bm == 128 && g.bn == 128) launch_tile<128, 128>(s, grid, g, w);
-  else if (g.bm == 128) launch_tile<128, 64>(s, grid, g, w);
-  else if (g.bn == 128) launch_tile<64, 128>(s, grid, g, w);
-  else launch_tile<64, 64>(s, grid, g, w);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || g.nsplit == 1) return e;
-  hipLaunchKernelGGL(gemm_splitk_reduce, dim3(grid_for(g.M * g.N, 256)), dim3(256), 0, s, g,
-                     g.nsplit, (const double*)ws);
+template <int RM, int RN, int WK>
+static void launch_reg(hipStream_t s, const GemmArgs& g) {
+  constexpr int64_t TBM = (4 / WK) * 16 * RM, TBN = 16 * RN;
+  dim3 grid((unsigned)cdiv(g.M, TBM), (unsigned)cdiv(g.N, TBN), (unsigned)g.nsplit);
+  hipLaunchKernelGGL((gemm_f64_reg<RM, RN, WK>), grid, dim3(256), 0, s, g);
+}
+
+// C^T = B^T A^T: the same product with the roles of the operands swapped.
+static GemmArgs transposed(const GemmArgs& g) {
+  GemmArgs t = g;
+  t.M = g.N;
+  t.N = g.M;
+  t.A = g.B;
+  t.rowA = g.colB;
+  t.kA = g.kB;
+  t.kA_aff = g.kB_aff;
+  t.B = g.A;
+  t.colB = g.rowA;
+  t.kB = g.kA;
+  t.kB_aff = g.kA_aff;
+  t.rowC = g.colC;
+  t.colC = g.rowC;
+  t.a_kfast = g.b_kfast;
+  t.b_kfast = g.a_kfast;
+  t.trans = 0;
+  return t;
+}
+
+hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
+  const GemmArgs g = g0.trans ? transposed(g0) : g0;
+  if (g.nsplit > 1 && (!g.ws || !g.counters)) return hipErrorInvalidValue;
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (g.kind == 1) {
+    const int key = g.rm * 100 + g.rn * 10 + g.wk;
+    switch (key) {
+      case 411: launch_reg<4, 1, 1>(s, g); break;
+      case 221: launch_reg<2, 2, 1>(s, g); break;
+      case 114: launch_reg<1, 1, 4>(s, g); break;
+      case 214: launch_reg<2, 1, 4>(s, g); break;
+      case 414: launch_reg<4, 1, 4>(s, g); break;
+      case 614: launch_reg<6, 1, 4>(s, g); break;
+      case 224: launch_reg<2, 2, 4>(s, g); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
+  dim3 grid((unsigned)cdiv(g.M, g.bm), (unsigned)cdiv(g.N, g.bn), (unsigned)g.nsplit);
+  if (g.bk == 32) {
+    if (g.bm == 128 && g.bn == 128) return hipErrorInvalidValue;
+    else if (g.bm == 128) launch_tile<128, 64, 32>(s, grid, g);
+    else if (g.bn == 128) launch_tile<64, 128, 32>(s, grid, g);
+    else launch_tile<64, 64, 32>(s, grid, g);
+  } else {
+    if (g.bm == 128 && g.bn == 128) launch_tile<128, 128, 16>(s, grid, g);
+    else if (g.bm == 128) launch_tile<128, 64, 16>(s, grid, g);
+    else if (g.bn == 128) launch_tile<64, 128, 16>(s, grid, g);
+    else launch_tile<64, 64, 16>(s, grid, g);
+  }
   return hipGetLastError();
 }
 
-hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A,
-                           const int64_t* rowA, const int64_t* colA, double* C,
-                           const int64_t* rowC, const int64_t* colC, double alpha,
-                           int accumulate) {
-  hipLaunchKernelGGL(perm_add, dim3(grid_for(M * N, 256)), dim3(256), 0, s, M, N, A, rowA, colA,
-                     C, rowC, colC, alpha, accumulate);
+hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
+                           const int* colA, double* C, const int* rowC, const int* colC,
+                           double alpha, int accumulate) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (N > 0x7fffffff) return hipErrorInvalidValue;
+  int shift = 0;
+  while ((1 << shift) < N && shift < 8) ++shift;
+  const int64_t rows_per_block = 256 >> shift;
+  hipLaunchKernelGGL(perm_add, dim3(grid_for(M, (int)rows_per_block, 8192)), dim3(256), 0, s, M,
+                     (int)N, shift, A, rowA, colA, C, rowC, colC, alpha, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather64(hipStream_t s, int64_t M, int64_t N, const double* X,
+                           const int64_t* rowX, const int64_t* colX, double* Y,
+                           const int64_t* rowY, const int64_t* colY) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather64, dim3((unsigned)std::min<int64_t>(M, 65536)), dim3(256), 0, s, M, N,
+                     X, rowX, colX, Y, rowY, colY);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_flat(hipStream_t s, int64_t n, const double* A, double* C, double alpha,
+                            int accumulate) {
+  if (n <= 0) return hipSuccess;
+  // double2 access needs 16-byte alignment of both arrays
+  if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(C)) & 15) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(axpy_flat, dim3(grid_for((n + 1) / 2, 256, 8192)), dim3(256), 0, s, n, A, C,
+                     alpha, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(hipStream_t s, double* y, const double* x, int64_t n) {
+  hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, y, x, n);
   return hipGetLastError();
 }
 

@@ -45,6 +45,9 @@ EXPORTS = {
     "ccq_hip_t_size": (ctypes.c_int64, [ctypes.c_void_p]),
     "ccq_hip_set_t": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_get_t": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_t4_size": (ctypes.c_int64, [ctypes.c_void_p]),
+    "ccq_hip_set_t4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_get_t4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_update": (ctypes.c_int, [ctypes.c_void_p]),
     "ccq_hip_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "ccq_hip_diis_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
@@ -94,6 +97,7 @@ class Engine:
         self.h = h
         self.calc_type = calc_type
         self.t_size = int(self.lib.ccq_hip_t_size(self.h))
+        self.t4_size = int(self.lib.ccq_hip_t4_size(self.h))
 
     def _check(self, rc: int, what: str):
         if rc != 0:
@@ -114,6 +118,15 @@ class Engine:
     def get_t(self) -> np.ndarray:
         out = np.empty(self.t_size)
         self._check(self.lib.ccq_hip_get_t(self.h, _ptr(out), out.size), "get_t")
+        return out
+
+    def set_t4(self, t4: np.ndarray):
+        t = np.ascontiguousarray(t4, dtype=np.float64)
+        self._check(self.lib.ccq_hip_set_t4(self.h, _ptr(t), t.size), "set_t4")
+
+    def get_t4(self) -> np.ndarray:
+        out = np.empty(self.t4_size)
+        self._check(self.lib.ccq_hip_get_t4(self.h, _ptr(out), out.size), "get_t4")
         return out
 
     def update(self):
@@ -182,6 +195,31 @@ def block_shapes(oa, ob, va, vb):
         't3aaa': (oa, oa, oa, va, va, va), 't3aab': (oa, oa, ob, va, va, vb),
         't3abb': (oa, ob, ob, va, vb, vb), 't3bbb': (ob, ob, ob, vb, vb, vb),
     }
+
+
+T4_BLOCKS = ['t4aaaa', 't4aaab', 't4aabb', 't4abbb', 't4bbbb']
+
+
+def t4_shapes(oa, ob, va, vb):
+    return {
+        't4aaaa': (oa, oa, oa, oa, va, va, va, va), 't4aaab': (oa, oa, oa, ob, va, va, va, vb),
+        't4aabb': (oa, oa, ob, ob, va, va, vb, vb), 't4abbb': (oa, ob, ob, ob, va, vb, vb, vb),
+        't4bbbb': (ob, ob, ob, ob, vb, vb, vb, vb),
+    }
+
+
+def split_t4(t4, oa, ob, va, vb) -> dict:
+    out, off = {}, 0
+    for name, shp in t4_shapes(oa, ob, va, vb).items():
+        n = int(np.prod(shp))
+        out[name] = t4[off:off + n].reshape(shp)
+        off += n
+    return out
+
+
+def join_t4(blocks: dict, oa, ob, va, vb) -> np.ndarray:
+    return np.concatenate([np.ascontiguousarray(blocks[n]).ravel()
+                           for n in t4_shapes(oa, ob, va, vb)])
 
 
 def split_tvec(t_vec, oa, ob, va, vb) -> dict:

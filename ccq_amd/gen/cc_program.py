@@ -74,15 +74,19 @@ def build(method: str, rhf: bool):
     for name, groups in (('D3', [(0, 1)]), ('D4', [(0, 1), (2, 3)]), ('D5', [(2, 3)]),
                          ('Hv', [(0, 1), (2, 3)])):
         d(name, 4, 2, groups, kind='dressed')
-    fac = fz.Factorizer(prog, sizes={'o': 10, 'v': 70})
     gen = {}
+    needed_h = set()
     for rank, tps in TPRODS[lvl].items():
+        # one factoriser per residual rank: each residual section defines every
+        # intermediate it reads
+        fac = fz.Factorizer(prog, sizes={'o': 10, 'v': 70}, prefix=f"X{rank}_")
         terms = wick.residual_terms(rank, tps)
         stm = []
         for t in terms:
             stm += fac.statements(t, f"R{rank}", rank)
         gen[rank] = [prog.stmt(s) for s in stm]
-    hv_blocks = sorted({fz.canonical_h_block(sp) for k, sp in fac.needed_h if k == 'v'})
+        needed_h |= fac.needed_h
+    hv_blocks = sorted({fz.canonical_h_block(sp) for k, sp in needed_h if k == 'v'})
     dress_stm = cp.dress_statements(prog) + fz.dressed_v_statements(prog, hv_blocks)
     r2 = cp.r2_statements(prog) + gen.get(2, [])
     r1 = cp.r1_statements(prog) + gen.get(1, [])

@@ -357,15 +357,46 @@ def compile_section(prog: Program, name: str, stmts, targets, fold_target=True,
     for st in stmts:
         allops += integrate(prog, st)
     needed = set(targets)
-    keep = []
-    # backward liveness over statement order
-    for op in reversed(allops):
-        if op.out in needed:
-            keep.append(op)
-            for k, _ in op.ins:
-                if prog.decls[k.name].kind in live_kinds:
-                    needed.add(k)
-    keep.reverse()
+    # liveness to a fixed point (a block may be read by an op listed before
+    # the ops that write it)
+    while True:
+        grew = False
+        for op in allops:
+            if op.out in needed:
+                for k, _ in op.ins:
+                    if prog.decls[k.name].kind in live_kinds and k not in needed:
+                        needed.add(k)
+                        grew = True
+        if not grew:
+            break
+    keep = [op for op in allops if op.out in needed]
+    # An intermediate block that no op writes is identically zero: every spin
+    # assignment of its defining contraction violates conservation on some
+    # factor (e.g. X[m a c i k l] = V[m a e i] T2[k l c e] with k, l alpha and
+    # c beta).  Ops reading it contribute nothing; drop them, to a fixed point.
+    while True:
+        written = {op.out for op in keep}
+        alive = [op for op in keep
+                 if all(prog.decls[k.name].kind not in live_kinds or k in written
+                        for k, _ in op.ins)]
+        if len(alive) == len(keep):
+            break
+        keep = alive
+    # blocks that only fed dropped ops are no longer needed
+    needed = set(targets)
+    while True:
+        grew = False
+        for op in keep:
+            if op.out in needed:
+                for k, _ in op.ins:
+                    if prog.decls[k.name].kind in live_kinds and k not in needed:
+                        needed.add(k)
+                        grew = True
+        if not grew:
+            break
+    keep = [op for op in keep if op.out in needed]
+    if not any(op.out in set(targets) for op in keep):
+        raise RuntimeError(f"section {name}: no op writes its targets")
     # fold residual images per target block
     out = []
     by_out = {}

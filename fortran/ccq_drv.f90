@@ -81,6 +81,7 @@ program ccq_drv
     select case (trim(calc_type))
     case ('CCSD'); level = 2
     case ('CCSDT'); level = 3
+    case ('CCSDTQ'); level = 4
     case default
         write(0, '(a)') 'ccq_drv: calc_type '//trim(calc_type)//' is not supported'
         call exit(1)

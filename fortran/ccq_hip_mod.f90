@@ -56,6 +56,25 @@ module ccq_hip_mod
             integer(c_int64_t), value :: t_size
         end function ccq_hip_get_t
 
+        integer(c_int64_t) function ccq_hip_t4_size(ctx) bind(c, name='ccq_hip_t4_size')
+            import :: c_int64_t, c_ptr
+            type(c_ptr), value :: ctx
+        end function ccq_hip_t4_size
+
+        integer(c_int) function ccq_hip_set_t4(ctx, t4, n) bind(c, name='ccq_hip_set_t4')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: t4(*)
+            integer(c_int64_t), value :: n
+        end function ccq_hip_set_t4
+
+        integer(c_int) function ccq_hip_get_t4(ctx, t4, n) bind(c, name='ccq_hip_get_t4')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: t4(*)
+            integer(c_int64_t), value :: n
+        end function ccq_hip_get_t4
+
         integer(c_int) function ccq_hip_update(ctx) bind(c, name='ccq_hip_update')
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

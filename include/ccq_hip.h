@@ -66,6 +66,14 @@ int64_t ccq_hip_t_size(const ccq_hip_ctx* ctx);
 int ccq_hip_set_t(ccq_hip_ctx* ctx, const double* t_vec, int64_t t_size);
 int ccq_hip_get_t(ccq_hip_ctx* ctx, double* t_vec, int64_t t_size);
 
+/* T4 (CCSDTQ) stays resident on the device; ccq keeps it in the files
+ * <bin_file>_4{a..e}.bin (src/utils/cc_utils.f90:89-145).  Blocks
+ * T4A..T4E (aaaa, aaab, aabb, abbb, bbbb) concatenated, each C order
+ * occupied-first.  Size 0 unless level == 4. */
+int64_t ccq_hip_t4_size(const ccq_hip_ctx* ctx);
+int ccq_hip_set_t4(ccq_hip_ctx* ctx, const double* t4, int64_t n);
+int ccq_hip_get_t4(ccq_hip_ctx* ctx, double* t4, int64_t n);
+
 /* One in-place amplitude sweep in ccq's order: replaces update_ptr =>
  * update_clusters_t4 (src/update_cc.F90:12-323). */
 int ccq_hip_update(ccq_hip_ctx* ctx);

@@ -8,14 +8,20 @@ from ccq_amd import engine as E
 from ccq_amd import synth
 
 
-def antisym_random(rng, shape_name, oa, ob, va, vb, scale=0.05):
-    import itertools
+def _groups(shape_name):
+    """Antisymmetric axis groups of a block: same-spin runs of occupied and of
+    virtual indices (t3aab: occupied (0, 1), virtual (3, 4))."""
     rank, nb = osolver.BLOCKS[shape_name]
-    x = scale * rng.standard_normal(osolver.block_shape(shape_name, oa, ob, va, vb))
     na = rank - nb
-    groups = [list(range(0, na)), list(range(na, rank)),
-              list(range(rank, rank + na)), list(range(rank + na, 2 * rank))]
-    for g in groups:
+    return [list(range(0, na)), list(range(na, rank)),
+            list(range(rank, rank + na)), list(range(rank + na, 2 * rank))]
+
+
+def antisymmetrize_block(x, shape_name):
+    """Project onto the antisymmetric subspace of the block's groups."""
+    import itertools
+    rank = osolver.BLOCKS[shape_name][0]
+    for g in _groups(shape_name):
         if len(g) < 2:
             continue
         acc = np.zeros_like(x)
@@ -27,6 +33,21 @@ def antisym_random(rng, shape_name, oa, ob, va, vb, scale=0.05):
             acc += osolver.bf._parity(p) * x.transpose(axes)
         x = acc / len(perms)
     return x
+
+
+def canonical_mask(shape_name, shape):
+    """True on tuples strictly increasing inside every antisymmetric group."""
+    idx = np.indices(shape)
+    mask = np.ones(shape, dtype=bool)
+    for g in _groups(shape_name):
+        for a, b in zip(g, g[1:]):
+            mask &= idx[a] < idx[b]
+    return mask
+
+
+def antisym_random(rng, shape_name, oa, ob, va, vb, scale=0.05):
+    x = scale * rng.standard_normal(osolver.block_shape(shape_name, oa, ob, va, vb))
+    return antisymmetrize_block(x, shape_name)
 
 
 class Case:
@@ -74,3 +95,20 @@ class Case:
         fa = self.fa[self.froz:, self.froz:]
         fb = self.fb[self.froz:, self.froz:]
         return osolver.energy(fa, fb, self.vaa, self.vab, self.vbb, self.ca, self.cb, t)
+
+
+def ccsdtq_sweep_case():
+    """Inputs of the stored CCSDTQ sweep fixture (golden/make_ccsdtq_sweep.py):
+    8 orbitals, 4 alpha / 3 beta occupied, random antisymmetric T1-T4."""
+    n, oa, ob = 8, 4, 3
+    s = synth.make_system(n, oa, 20, 9)
+    e2 = s.e2int()
+    fa, fb = I.gen_fock(s.h, e2, oa, ob)
+    vaa, vab, vbb = I.antisymmetrize(e2, 0)
+    va, vb = n - oa, n - ob
+    rng = np.random.default_rng(2)
+    t = osolver.zero_amplitudes('CCSDTQ', oa, ob, va, vb)
+    for k in t:
+        t[k] = antisym_random(rng, k, oa, ob, va, vb, 0.03)
+    return dict(n=n, oa=oa, ob=ob, va=va, vb=vb, h=s.h, eri=s.eri, fa=fa, fb=fb, vaa=vaa,
+                vab=vab, vbb=vbb, t=t, shift=0.05)

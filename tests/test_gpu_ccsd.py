@@ -114,3 +114,38 @@ def test_engine_error_paths():
         eng.diis_record(it)
     with pytest.raises(E.EngineError, match="DGESV"):
         eng.diis_extrapolate()
+
+
+@pytest.mark.parametrize("method", ["CCSD", "CCSDT"])
+def test_graph_replays_match_eager(method, monkeypatch):
+    """The captured sweep graph, replayed several times with eager work and
+    synchronisations in between, equals the eagerly launched sweep each time."""
+    _say(f"case graph replay {method}")
+    from oracle import solver as osolver
+    from gpu_helpers import antisym_random
+    c = Case(7, 3, 20, 61, occ_b=2)
+    rng = np.random.default_rng(4)
+    t = osolver.zero_amplitudes(method, c.ca, c.cb, c.va, c.vb)
+    lvl = {"CCSD": 2, "CCSDT": 3}[method]
+    for k in t:
+        if osolver.BLOCKS[k][0] <= lvl:
+            t[k] = antisym_random(rng, k, c.ca, c.cb, c.va, c.vb, 0.05)
+    vec = E.join_tvec(t, c.ca, c.cb, c.va, c.vb)
+    engines = []
+    for graph in ("1", "0"):
+        monkeypatch.setenv("CCQ_HIP_GRAPH", graph)
+        eng = E.Engine(0, c.oa, c.ob, c.n, method, shift=0.03)
+        eng.set_ints(c.fa, c.fb, c.vaa, c.vab, c.vbb)
+        eng.set_t(vec)
+        engines.append(eng)
+    for rep in range(3):
+        out = []
+        for eng in engines:
+            eng.update()
+            e = eng.energy()
+            eng.diis_record(rep + 1)
+            out.append((e, eng.get_t()))
+        assert abs(out[0][0] - out[1][0]) < 1e-13
+        np.testing.assert_array_equal(out[0][1], out[1][1])
+    for eng in engines:
+        eng.close()

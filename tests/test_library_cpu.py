@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     text = open(os.path.join(ROOT, "include", "ccq_hip.h")).read()
-    return sorted(set(re.findall(r"\b(ccq_hip_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(ccq_hip_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -32,6 +32,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_embedded_ir_is_current():
+    gh = emit.generator_hash()
+    for method in emit.METHODS:
+        for tag in ("uhf", "rhf"):
+            path = os.path.join(ROOT, "ccq_amd", "csrc", f"ir_{method.lower()}_{tag}.inc")
+            with open(path) as f:
+                assert f"generator {gh}" in f.readline(), path
     for rhf in (False, True):
         prog, secs = ccsd_program.build(rhf=rhf)
         text = emit.emit(prog, secs, "CCSD", rhf)
