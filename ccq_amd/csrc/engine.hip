@@ -149,7 +149,6 @@ struct ccq_hip_ctx {
   struct Gather { int t; int64_t rows, cols, t_src_row, t_src_col, t_dst_row, t_dst_col; };
   std::vector<Gather> gathers;
   DevBuf ws;                 // split-K workspace
-  unsigned* counters = nullptr;   // split-K tile tickets (kept zero between launches)
   int* itab = nullptr;
   double* dtab = nullptr;
   // program
@@ -166,7 +165,6 @@ struct ccq_hip_ctx {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (tables32) (void)hipFree(tables32);
-    if (counters) (void)hipFree(counters);
     if (gather_tab) (void)hipFree(gather_tab);
     if (itab) (void)hipFree(itab);
     if (dtab) (void)hipFree(dtab);
@@ -465,7 +463,6 @@ struct ccq_hip_ctx {
   }
 
   int64_t ws_need = 0;
-  int64_t n_counters = 0;
 
   // k * stride for every k (the offset table of a summed index run that is
   // contiguous-by-stride in memory), or -1
@@ -569,7 +566,6 @@ struct ccq_hip_ctx {
           g.kB_aff = affine_stride(kB);
           ccqk::gemm_plan(g, kWsCap);
           ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(g));
-          if (g.nsplit > 1) n_counters = std::max<int64_t>(n_counters, ccqk::gemm_tiles(g));
           op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +
                    std::to_string(op.K) + "/" + plan_tag(g);
           op.flops = 2.0 * (double)op.M * (double)op.N * (double)(ks.empty() ? 1 : op.K);
@@ -595,8 +591,6 @@ struct ccq_hip_ctx {
       }
     }
     ws.alloc(std::max<int64_t>(ws_need, 1));
-    HIPCHK(hipMalloc(&counters, sizeof(unsigned) * std::max<int64_t>(n_counters, 1)));
-    HIPCHK(hipMemset(counters, 0, sizeof(unsigned) * std::max<int64_t>(n_counters, 1)));
     HIPCHK(hipMalloc(&tables32, sizeof(int) * std::max<size_t>(host32.size(), 1)));
     if (!host32.empty())
       HIPCHK(hipMemcpy(tables32, host32.data(), sizeof(int) * host32.size(), hipMemcpyHostToDevice));
@@ -629,7 +623,6 @@ struct ccq_hip_ctx {
       g.alpha = op.coef;
       g.accumulate = acc;
       g.ws = ws.p;
-      g.counters = counters;
       HIPCHK(ccqk::launch_gemm(stream, g));
     } else {
       const bool aligned =

@@ -34,10 +34,8 @@ struct GemmArgs {
   int rm = 1, rn = 1, wk = 1;         // register-direct wave tile / waves along K
   int nsplit = 1;
   int64_t kchunk = 0;
-  // split-K state: ws >= gemm_ws_need doubles, counters >= gemm_tiles
-  // zero-initialised unsigned ints (every launch leaves them zero)
+  // split-K workspace of at least gemm_ws_need doubles
   double* ws = nullptr;
-  unsigned* counters = nullptr;
 };
 
 struct UpdateParams {
@@ -71,8 +69,8 @@ struct Coeffs {
 // Choose kernel, tile shape and split-K for a GEMM; ws_cap bounds the
 // split-K workspace (doubles).
 void gemm_plan(GemmArgs& g, int64_t ws_cap);
-// Output tiles of the planned launch, and split-K workspace it needs (0 when
-// nsplit == 1).
+// Output tiles of the planned launch, and split-K workspace (doubles) it
+// needs (0 when nsplit == 1).
 int64_t gemm_tiles(const GemmArgs& g);
 int64_t gemm_ws_need(const GemmArgs& g);
 hipError_t launch_gemm(hipStream_t s, const GemmArgs& g);

@@ -58,62 +58,11 @@ __device__ __forceinline__ double sum_strided(const double* __restrict__ q, int6
   return (a0 + a1) + (a2 + a3);
 }
 
-// Split-K completion, shared by both GEMM kernels.  Each of the nsplit
-// workgroups of an output tile has stored its partial tile (TBM x TBN,
-// row-major) at ws + (tile * nsplit + z) * TBM * TBN and calls splitk_last:
-// it publishes the partial (release fence) and takes a ticket on the tile's
-// counter.  The workgroup holding the last ticket resets the counter for the
-// next launch, acquires, and runs splitk_finish, which sums the partials in an
-// order fixed by (nsplit, tile extent) -- the result does not depend on which
-// workgroup arrived last -- and writes C.  No workgroup ever waits on another.
-constexpr int kRed = 2048;   // LDS doubles splitk_finish may use
-
-__device__ __forceinline__ bool splitk_last(const GemmArgs& g, int tile) {
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned prev = atomicAdd(g.counters + tile, 1u);
-    last = prev == (unsigned)(g.nsplit - 1) ? 1 : 0;
-    if (last) g.counters[tile] = 0u;
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  return true;
-}
-
-template <int TBM, int TBN>
-__device__ void splitk_finish(const GemmArgs& g, const double* __restrict__ part, int64_t m0,
-                              int64_t n0, double* red) {
-  constexpr int TE = TBM * TBN;
-  const int mv = (int)(g.M - m0 < TBM ? g.M - m0 : TBM);
-  const int nv = (int)(g.N - n0 < TBN ? g.N - n0 : TBN);
-  const int E = mv * nv;
-  const int ns = g.nsplit;
-  // P threads per output element when the tile has few valid elements
-  int P = 1;
-  while (2 * P <= ns && E * 2 * P <= kRed) P *= 2;
-  const int tid = threadIdx.x;
-  if (P == 1) {
-    for (int e = tid; e < E; e += 256) {
-      const int r = e / nv, c = e - r * nv;
-      store_c(g, m0 + r, n0 + c, sum_strided(part + r * TBN + c, (int64_t)TE, ns));
-    }
-    return;
-  }
-  for (int w = tid; w < E * P; w += 256) {
-    const int e = w / P, p = w - e * P;
-    const int r = e / nv, c = e - r * nv;
-    red[w] = sum_strided(part + (int64_t)p * TE + r * TBN + c, (int64_t)P * TE, (ns - p + P - 1) / P);
-  }
-  __syncthreads();
-  for (int e = tid; e < E; e += 256) {
-    double s = 0.0;
-    for (int p = 0; p < P; ++p) s += red[e * P + p];
-    const int r = e / nv, c = e - r * nv;
-    store_c(g, m0 + r, n0 + c, s);
-  }
+// Split-K: workgroup z of a tile stores its partial product of K range
+// [z*kchunk, (z+1)*kchunk) at ws[z][m][n] (M x N, row-major); gemm_splitk_reduce
+// sums the nsplit partials of every element in a fixed order and writes C.
+__device__ __forceinline__ void store_partial(const GemmArgs& g, int64_t m, int64_t n, double v) {
+  g.ws[((int64_t)blockIdx.z * g.M + m) * g.N + n] = v;
 }
 
 // LDS tile of an operand: R x BK values.  KF (k-fastest in global memory):
@@ -175,17 +124,15 @@ struct Loader {
 // (BM/2) x (BN/2) sub-tile of 16x16 MFMA tiles.  Operands are gathered through
 // int32 offset tables (rowA/kA, kB/colB); the next K slab is fetched into
 // registers while the current one is multiplied out of LDS.  blockIdx.z owns
-// K range [z*kchunk, (z+1)*kchunk) (split-K, completed by splitk_finish).
+// K range [z*kchunk, (z+1)*kchunk) (split-K, completed by gemm_splitk_reduce).
 template <int BM, int BN, int BK, bool AKF, bool BKF>
 __global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   using TA = LdsTile<BM, AKF, BK>;
   using TB = LdsTile<BN, BKF, BK>;
-  constexpr int SMEM = TA::size + TB::size > kRed ? TA::size + TB::size : kRed;
-  __shared__ double smem[SMEM];
-  double* As = smem;
-  double* Bs = smem + TA::size;
+  __shared__ double As[TA::size];
+  __shared__ double Bs[TB::size];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -241,32 +188,20 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g) {
 
   // D layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l, register v
   // holds row (l/16) + 4*v, column l%16
-  if (g.nsplit == 1) {
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-      const int64_t gn = n0 + wn + 16 * j + (lane & 15);
-      if (gn >= g.N) continue;
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int64_t gm = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
-          if (gm < g.M) store_c(g, gm, gn, acc[i][j][v]);
-        }
-    }
-    return;
-  }
-  const int tile = blockIdx.x + gridDim.x * blockIdx.y;
-  double* part = g.ws + ((int64_t)tile * g.nsplit + blockIdx.z) * (BM * BN);
-#pragma unroll
-  for (int j = 0; j < TN; ++j)
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gn = n0 + wn + 16 * j + (lane & 15);
+    if (gn >= g.N) continue;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        part[(wm + 16 * i + (lane >> 4) + 4 * v) * BN + wn + 16 * j + (lane & 15)] = acc[i][j][v];
-  if (!splitk_last(g, tile)) return;
-  splitk_finish<BM, BN>(g, g.ws + (int64_t)tile * g.nsplit * (BM * BN), m0, n0, smem);
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gm = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
+        if (gm >= g.M) continue;
+        if (g.nsplit == 1) store_c(g, gm, gn, acc[i][j][v]);
+        else store_partial(g, gm, gn, acc[i][j][v]);
+      }
+  }
 }
 
 // Register-direct GEMM for skinny and dot-shaped products (an output extent
@@ -316,7 +251,7 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
   constexpr int WMW = 4 / WK;
   constexpr int TBM = WMW * 16 * RM, TBN = 16 * RN;
   constexpr int WACC = 256 * RM * RN;   // doubles of one wave's accumulators
-  constexpr int SMEM = (WK - 1) * WMW * WACC > kRed ? (WK - 1) * WMW * WACC : kRed;
+  constexpr int SMEM = WK > 1 ? (WK - 1) * WMW * WACC : 1;
   __shared__ double red[SMEM];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wk = wave % WK, wmi = wave / WK;
@@ -382,35 +317,47 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
     }
   }
 
-  if (g.nsplit == 1) {
-    if (wk != 0) return;
+  if (wk != 0) return;
 #pragma unroll
-    for (int j = 0; j < RN; ++j) {
-      const int64_t gn = n0 + 16 * j + r;
-      if (gn >= g.N) continue;
+  for (int j = 0; j < RN; ++j) {
+    const int64_t gn = n0 + 16 * j + r;
+    if (gn >= g.N) continue;
 #pragma unroll
-      for (int i = 0; i < RM; ++i)
+    for (int i = 0; i < RM; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int64_t gm = wm0 + 16 * i + q + 4 * v;
-          if (gm < g.M) store_c(g, gm, gn, acc[i][j][v]);
-        }
-    }
-    return;
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gm = wm0 + 16 * i + q + 4 * v;
+        if (gm >= g.M) continue;
+        if (g.nsplit == 1) store_c(g, gm, gn, acc[i][j][v]);
+        else store_partial(g, gm, gn, acc[i][j][v]);
+      }
   }
-  const int tile = blockIdx.x + gridDim.x * blockIdx.y;
-  if (wk == 0) {
-    double* part = g.ws + ((int64_t)tile * g.nsplit + blockIdx.z) * (TBM * TBN);
-#pragma unroll
-    for (int j = 0; j < RN; ++j)
-#pragma unroll
-      for (int i = 0; i < RM; ++i)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-          part[(wmi * 16 * RM + 16 * i + q + 4 * v) * TBN + 16 * j + r] = acc[i][j][v];
+}
+
+// C (+)= alpha * sum_z ws[z] for split-K GEMMs.  P = 2^pshift threads share an
+// element (z = p, p + P, ...), combined through LDS in p order: the summation
+// order depends only on nsplit, so results are deterministic.
+__global__ __launch_bounds__(256) void gemm_splitk_reduce(GemmArgs g, int pshift) {
+  __shared__ double red[256];
+  const int P = 1 << pshift;
+  const int64_t total = g.M * g.N;
+  const int p = threadIdx.x & (P - 1);
+  const int64_t e = (int64_t)blockIdx.x * (256 >> pshift) + (threadIdx.x >> pshift);
+  double s = 0.0;
+  if (e < total) {
+    const int cnt = (g.nsplit - p + P - 1) / P;
+    s = sum_strided(g.ws + (int64_t)p * total + e, (int64_t)P * total, cnt);
   }
-  if (!splitk_last(g, tile)) return;
-  splitk_finish<TBM, TBN>(g, g.ws + (int64_t)tile * g.nsplit * (TBM * TBN), m0, n0, red);
+  if (P > 1) {
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (p != 0) return;
+    for (int x = 1; x < P; ++x) s += red[threadIdx.x + x];
+  }
+  if (e < total) {
+    const int64_t m = e / g.N, n = e - m * g.N;
+    store_c(g, m, n, s);
+  }
 }
 
 // C(perm) (+)= alpha A through int32 offset tables: element (m, n) reads
@@ -661,8 +608,8 @@ static int grid_for(int64_t n, int threads, int cap = 4096) {
 // Register-direct kernel variants: (RM, RN, WK).  Output tile
 // ((4/WK) * 16 * RM) x (16 * RN).
 struct RegShape { int rm, rn, wk; };
-static const RegShape kRegShapes[] = {{4, 1, 1}, {2, 2, 1}, {1, 1, 4}, {2, 1, 4},
-                                      {4, 1, 4}, {6, 1, 4}, {2, 2, 4}};
+static const RegShape kRegShapes[] = {{1, 1, 1}, {2, 1, 1}, {4, 1, 1}, {2, 2, 1}, {1, 1, 4},
+                                      {2, 1, 4}, {4, 1, 4}, {6, 1, 4}, {2, 2, 4}};
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -683,25 +630,14 @@ int64_t gemm_tiles(const GemmArgs& g) {
   return cdiv(M, tbm) * cdiv(N, tbn);
 }
 
-int64_t gemm_ws_need(const GemmArgs& g) {
-  if (g.nsplit <= 1) return 0;
-  int64_t tbm, tbn;
-  tile_dims(g, tbm, tbn);
-  return gemm_tiles(g) * g.nsplit * tbm * tbn;
-}
+int64_t gemm_ws_need(const GemmArgs& g) { return g.nsplit > 1 ? g.nsplit * g.M * g.N : 0; }
 
-// Split-K completion: every workgroup stores its partial tile; the last one
-// of a tile reads all nsplit partials (~64 B/cycle into one CU, a few
-// latency-bound rounds) and pays a fence/atomic round trip.
-static double splitk_cycles(int64_t tiles, int64_t split, int64_t tile_elems, int64_t valid_elems) {
+// Split-K: the partials make one HBM round trip and the reduction is one more
+// launch (~2 us inside a graph).
+static double splitk_cycles(int64_t M, int64_t N, int64_t split) {
   if (split <= 1) return 0.0;
   const double hbm = 8.0e12 / 2.4e9;
-  const double write = (double)tiles * split * tile_elems * 8.0 / hbm;
-  int64_t P = 1;
-  while (2 * P <= split && valid_elems * 2 * P <= 2048) P *= 2;
-  const double per_thread = (double)cdiv(valid_elems * P, 256) * cdiv(split, P);
-  const double read = (double)valid_elems * split * 8.0 / 64.0;
-  return write + std::max(read, per_thread / 4.0 * 700.0) + 4000.0;
+  return 2.0 * split * M * N * 8.0 / hbm + 5000.0;
 }
 
 void gemm_plan(GemmArgs& g, int64_t ws_cap) {
@@ -738,25 +674,25 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
     for (const Shape& sh : shapes) {
       if (bk == 32 && sh.bm == 128 && sh.bn == 128) continue;   // exceeds 256 VGPRs
       const int64_t tiles = cdiv(g.M, sh.bm) * cdiv(g.N, sh.bn);
-      const int64_t valid = std::min<int64_t>(g.M, sh.bm) * std::min<int64_t>(g.N, sh.bn);
       const double slab = (sh.bm / 32.0) * (sh.bn / 32.0) * (bk / 4.0) * 64.0;
       const int64_t maxsplit = std::max<int64_t>(1, std::min<int64_t>(ksteps / 2, 512));
       for (int64_t split = 1; split <= maxsplit; split *= 2) {
-        if (split > 1 && tiles * split * sh.bm * sh.bn > ws_cap) break;
+        if (split > 1 && split * g.M * g.N > ws_cap) break;
         const int64_t steps = cdiv(ksteps, split);
         const double per_cu = std::ceil((double)tiles * split / cus);
         const double resident = std::min<double>(per_cu, sh.occ);
         double cyc = per_cu * (steps + 2) * (slab + 300.0);
         if (resident < 2) cyc += per_cu * steps * 1500.0;
-        cyc += splitk_cycles(tiles, split, (int64_t)sh.bm * sh.bn, valid);
+        cyc += splitk_cycles(g.M, g.N, split);
         take(cyc, 0, 0, sh.bm, sh.bn, bk, nullptr, split, steps * bk);
       }
     }
   }
 
   // Register-direct kernel for products with a skinny output side (N, or M
-  // when run transposed).  Per 16-deep k step a wave issues 4*RM*RN MFMAs and
-  // waits on one load round trip at most partially hidden by the prefetch;
+  // when run transposed).  A wave walks its 16-deep k steps with one step of
+  // prefetch, so a step costs a load round trip (~3000 cycles) unless enough
+  // resident waves share the SIMD to keep its MFMA pipe busy instead.
   // A is re-read once per tile column and B once per tile row.
   for (int trans = 0; trans < 2; ++trans) {
     const int64_t M = trans ? g.N : g.M, N = trans ? g.M : g.N;
@@ -764,18 +700,21 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
       const int64_t tbm = (4 / r.wk) * 16 * r.rm, tbn = 16 * r.rn;
       if (N > 2 * tbn) continue;
       const int64_t tiles = cdiv(M, tbm) * cdiv(N, tbn);
-      const int64_t valid = std::min<int64_t>(M, tbm) * std::min<int64_t>(N, tbn);
       const int64_t k16 = cdiv(g.K, 16);
-      const double step = std::max(4.0 * r.rm * r.rn * 64.0, 700.0);
+      const double mfma = 4.0 * r.rm * r.rn * 64.0;
+      const double vgpr = 40.0 + 24.0 * (r.rm + r.rn) + 8.0 * r.rm * r.rn;
+      const double occ = std::min(8.0, std::floor(512.0 / vgpr));
       const double bytes = 8.0 * ((double)M * g.K * cdiv(N, tbn) + (double)g.K * N * cdiv(M, tbm));
       for (int64_t split = 1; split <= std::min<int64_t>(k16, 1024); split *= 2) {
-        if (split > 1 && tiles * split * tbm * tbn > ws_cap) break;
-        const int64_t steps = cdiv(k16, split);
-        const double rounds = std::ceil((double)tiles * split / (cus * 4.0));
-        double cyc = rounds * ((double)cdiv(steps, r.wk) * step + 1500.0);
+        if (split > 1 && split * g.M * g.N > ws_cap) break;
+        const int64_t steps = cdiv(cdiv(k16, split), r.wk);
+        const double wps = (double)tiles * split * 4.0 / (cus * 4.0);   // waves per SIMD
+        const double rounds = std::ceil(wps / occ);
+        const double busy = std::min(wps, occ);
+        double cyc = rounds * (steps + 1) * std::max(3000.0, mfma * busy);
         cyc = std::max(cyc, bytes / hbm);
-        cyc += splitk_cycles(tiles, split, tbm * tbn, valid);
-        take(cyc, 1, trans, 64, 64, 16, &r, split, steps * 16);
+        cyc += splitk_cycles(g.M, g.N, split);
+        take(cyc, 1, trans, 64, 64, 16, &r, split, cdiv(k16, split) * 16);
       }
     }
   }
@@ -821,13 +760,12 @@ static GemmArgs transposed(const GemmArgs& g) {
   return t;
 }
 
-hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
-  const GemmArgs g = g0.trans ? transposed(g0) : g0;
-  if (g.nsplit > 1 && (!g.ws || !g.counters)) return hipErrorInvalidValue;
-  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+static hipError_t launch_kernels(hipStream_t s, const GemmArgs& g) {
   if (g.kind == 1) {
     const int key = g.rm * 100 + g.rn * 10 + g.wk;
     switch (key) {
+      case 111: launch_reg<1, 1, 1>(s, g); break;
+      case 211: launch_reg<2, 1, 1>(s, g); break;
       case 411: launch_reg<4, 1, 1>(s, g); break;
       case 221: launch_reg<2, 2, 1>(s, g); break;
       case 114: launch_reg<1, 1, 4>(s, g); break;
@@ -851,6 +789,22 @@ hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
     else if (g.bn == 128) launch_tile<64, 128, 16>(s, grid, g);
     else launch_tile<64, 64, 16>(s, grid, g);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
+  const GemmArgs g = g0.trans ? transposed(g0) : g0;
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (g.nsplit > 1 && !g.ws) return hipErrorInvalidValue;
+  hipError_t e = launch_kernels(s, g);
+  if (e != hipSuccess || g.nsplit == 1) return e;
+  // threads per element: enough to cover nsplit in a few loads each
+  int pshift = 0;
+  while ((1 << pshift) < g.nsplit && (1 << pshift) < 32 && g.M * g.N * (2 << pshift) <= 262144)
+    ++pshift;
+  const int64_t per_block = 256 >> pshift;
+  hipLaunchKernelGGL(gemm_splitk_reduce, dim3((unsigned)cdiv(g.M * g.N, per_block)), dim3(256), 0,
+                     s, g, pshift);
   return hipGetLastError();
 }
 
