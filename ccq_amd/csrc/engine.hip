@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -25,6 +26,7 @@
 #include "ir_ccsd_uhf.inc"
 #include "ir_ccsdt_rhf.inc"
 #include "ir_ccsdt_uhf.inc"
+#include "ir_ccsdt_act_uhf.inc"
 #if __has_include("ir_ccsdtq_uhf.inc") && __has_include("ir_ccsdtq_rhf.inc")
 #include "ir_ccsdtq_rhf.inc"
 #include "ir_ccsdtq_uhf.inc"
@@ -104,6 +106,8 @@ struct UpdateStep {
   std::vector<double> signs;
   int dev_perm_off = 0;      // offset into the int table
   int dev_sign_off = 0;
+  bool masked = false;       // active-space triples: update only the masked tuples
+  int vneed = 0, vforbid = 0;
 };
 
 struct Section {
@@ -135,9 +139,26 @@ struct ccq_hip_ctx {
   // amplitude layout (ccq t_vec)
   std::map<std::string, int64_t> blk_off, blk_size;
   std::map<std::string, std::vector<int64_t>> blk_dims;
-  int64_t t_size = 0;        // ccq's pos(10)-1
+  int64_t t_size = 0;        // full-storage T1..T3 (ccq's pos(10)-1)
   int64_t dev_t = 0;         // doubles of t_vec kept on the device
   int64_t vec_size = 0;      // DIIS vector length
+  int64_t ext_size = 0;      // ccq's t_vec (t_size, or pos(16)-1 for CCSDt)
+  // active-space CCSDt: T3 is held in full storage with its Q-space tuples
+  // zero; ccq's t_vec packs the P space into ten blocks
+  bool active = false;
+  int64_t act_occ = -1, act_unocc = -1;
+  int64_t occ_start = 0;     // first active occupied index (both spins)
+  int64_t vir_end[2] = {0, 0};   // active virtuals per spin (the lowest ones)
+  struct PackPiece {         // one packed block, or one antisymmetric image of it
+    int64_t rows, cols, packed_off;
+    int64_t t_full_row, t_full_col, t_pk_row, t_pk_col;   // offsets into pk_tab
+    double sign;
+    bool image;              // unpack-only image (pack uses the identity pieces)
+    int blk;                 // index into pack_block
+  };
+  std::vector<PackPiece> pack_pieces;
+  int64_t* pk_tab = nullptr;
+  DevBuf staging;            // ccq-layout t_vec on the device
   // device state
   DevBuf Vaa, Vab, Vbb, Fa, Fb, fdiag_a, fdiag_b, tvec, ring, partial;
   double* vbb_ptr = nullptr;   // Vbb, or Vaa when ccq's v_bb equals v_aa
@@ -165,6 +186,7 @@ struct ccq_hip_ctx {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (tables32) (void)hipFree(tables32);
+    if (pk_tab) (void)hipFree(pk_tab);
     if (gather_tab) (void)hipFree(gather_tab);
     if (itab) (void)hipFree(itab);
     if (dtab) (void)hipFree(dtab);
@@ -214,7 +236,165 @@ struct ccq_hip_ctx {
       if (std::string(b.name) == "t3bbb") t_size = off;
     }
     vec_size = (level == 2) ? blk_off["t3aaa"] : t_size;
-    dev_t = (level == 2) ? blk_off["t3aaa"] : (level == 3 ? t_size : off);
+    dev_t = (level == 2) ? blk_off["t3aaa"] : (level == 4 ? off : t_size);
+    ext_size = t_size;
+    if (active) {
+      plan_packing();
+      vec_size = ext_size;
+    }
+  }
+
+  // ccq's CCSDt t_vec: T1, T2, then ten P-space T3 blocks with the index
+  // ranges of src/cc_opt_ccsdt/t2B0000_update_opt.f:48-57 (sizes
+  // get_t_sizes_act, src/utils/cc_utils.f90:560-631).  Each range is given
+  // per Fortran slot (fastest first) as (space, spin, part).
+  void plan_packing() {
+    enum Part { All, Act, Inact };
+    struct Slot { char space; int spin; Part part; };
+    struct PB { const char* block; Slot s[6]; };
+    const PB pbs[10] = {
+        {"t3aaa", {{'v', 0, All}, {'v', 0, All}, {'v', 0, Act}, {'o', 0, All}, {'o', 0, All}, {'o', 0, Act}}},
+        {"t3aab", {{'v', 1, All}, {'v', 0, All}, {'v', 0, Act}, {'o', 1, All}, {'o', 0, All}, {'o', 0, Act}}},
+        {"t3aab", {{'v', 1, Act}, {'v', 0, Inact}, {'v', 0, Inact}, {'o', 1, Act}, {'o', 0, Inact}, {'o', 0, Inact}}},
+        {"t3aab", {{'v', 1, Act}, {'v', 0, Inact}, {'v', 0, Inact}, {'o', 1, All}, {'o', 0, All}, {'o', 0, Act}}},
+        {"t3aab", {{'v', 1, All}, {'v', 0, All}, {'v', 0, Act}, {'o', 1, Act}, {'o', 0, Inact}, {'o', 0, Inact}}},
+        {"t3abb", {{'v', 1, All}, {'v', 1, Act}, {'v', 0, All}, {'o', 1, All}, {'o', 1, Act}, {'o', 0, All}}},
+        {"t3abb", {{'v', 1, Inact}, {'v', 1, Inact}, {'v', 0, Act}, {'o', 1, Inact}, {'o', 1, Inact}, {'o', 0, Act}}},
+        {"t3abb", {{'v', 1, Inact}, {'v', 1, Inact}, {'v', 0, Act}, {'o', 1, All}, {'o', 1, Act}, {'o', 0, All}}},
+        {"t3abb", {{'v', 1, All}, {'v', 1, Act}, {'v', 0, All}, {'o', 1, Inact}, {'o', 1, Inact}, {'o', 0, Act}}},
+        {"t3bbb", {{'v', 1, All}, {'v', 1, All}, {'v', 1, Act}, {'o', 1, All}, {'o', 1, All}, {'o', 1, Act}}},
+    };
+    auto range = [&](const Slot& sl, int64_t& lo, int64_t& hi) {
+      if (sl.space == 'o') {
+        const int64_t n = sl.spin ? ob : oa;
+        const int64_t st = std::min(occ_start, n);
+        lo = sl.part == Act ? st : 0;
+        hi = sl.part == Inact ? st : n;
+      } else {
+        const int64_t n = sl.spin ? vb : va;
+        const int64_t e = std::min(vir_end[sl.spin], n);
+        lo = sl.part == Inact ? e : 0;
+        hi = sl.part == Act ? e : n;
+      }
+    };
+    std::vector<int64_t> host;
+    int64_t off = blk_off.at("t3aaa");
+    pack_block.clear();
+    for (const PB& pb : pbs) {
+      pack_block.push_back(pb.block);
+      const std::string name = pb.block;
+      const auto& d = blk_dims.at(name);
+      // C-order strides of the full block; Fortran slot s is axis 5 - s
+      int64_t str[6];
+      str[5] = 1;
+      for (int a = 4; a >= 0; --a) str[a] = str[a + 1] * d[a + 1];
+      int64_t lo[6], hi[6], len[6];
+      int64_t count = 1;
+      for (int sl = 0; sl < 6; ++sl) {
+        range(pb.s[sl], lo[sl], hi[sl]);
+        len[sl] = std::max<int64_t>(0, hi[sl] - lo[sl]);
+        count *= len[sl];
+      }
+      // antisymmetric groups of the block (axes): same-spin occupied / virtual runs
+      const int nb = name == "t3aaa" ? 0 : name == "t3aab" ? 1 : name == "t3abb" ? 2 : 3;
+      const int na = 3 - nb;
+      std::vector<std::vector<int>> groups;
+      for (auto g : {std::vector<int>{0, 1, 2}, std::vector<int>{3, 4, 5}}) {
+        std::vector<int> ga(g.begin(), g.begin() + na), gb(g.begin() + na, g.end());
+        if (ga.size() > 1) groups.push_back(ga);
+        if (gb.size() > 1) groups.push_back(gb);
+      }
+      // every combination of within-group permutations: image q[a] = p[perm[a]]
+      std::vector<std::pair<std::vector<int>, double>> images;
+      std::function<void(size_t, std::vector<int>, double)> gen = [&](size_t gi, std::vector<int> perm,
+                                                                      double sign) {
+        if (gi == groups.size()) {
+          images.push_back({perm, sign});
+          return;
+        }
+        std::vector<int> g = groups[gi], img = g;
+        do {
+          std::vector<int> p2 = perm;
+          for (size_t k = 0; k < g.size(); ++k) p2[g[k]] = img[k];
+          // parity of img relative to g
+          int inv = 0;
+          for (size_t x = 0; x < img.size(); ++x)
+            for (size_t y = x + 1; y < img.size(); ++y)
+              if (img[x] > img[y]) ++inv;
+          gen(gi + 1, p2, (inv & 1) ? -sign : sign);
+        } while (std::next_permutation(img.begin(), img.end()));
+      };
+      gen(0, {0, 1, 2, 3, 4, 5}, 1.0);
+      const int64_t cols = len[0], rows = cols ? count / cols : 0;
+      for (size_t ii = 0; ii < images.size(); ++ii) {
+        const auto& perm = images[ii].first;
+        // position p (axis values) lands at q with q[a] = p[perm[a]]:
+        // offset = sum_b p[b] * str[perm^-1 b]
+        int64_t pstr[6];
+        for (int a = 0; a < 6; ++a) pstr[perm[a]] = str[a];
+        PackPiece pc{};
+        pc.rows = rows;
+        pc.cols = cols;
+        pc.packed_off = off;
+        pc.sign = images[ii].second;
+        pc.image = ii != 0;
+        pc.blk = (int)pack_block.size() - 1;
+        pc.t_full_row = (int64_t)host.size();
+        // rows: Fortran slots 1..5, slot 1 fastest
+        std::vector<int64_t> r(6, 0);
+        for (int64_t m = 0; m < rows; ++m) {
+          int64_t o = 0;
+          for (int sl = 1; sl < 6; ++sl) o += (lo[sl] + r[sl]) * pstr[5 - sl];
+          host.push_back(o);
+          for (int sl = 1; sl < 6; ++sl) {
+            if (++r[sl] < len[sl]) break;
+            r[sl] = 0;
+          }
+        }
+        pc.t_full_col = (int64_t)host.size();
+        for (int64_t n = 0; n < cols; ++n) host.push_back((lo[0] + n) * pstr[5]);
+        pc.t_pk_row = (int64_t)host.size();
+        for (int64_t m = 0; m < rows; ++m) host.push_back(m * cols);
+        pc.t_pk_col = (int64_t)host.size();
+        for (int64_t n = 0; n < cols; ++n) host.push_back(n);
+        // the full-block base offset is added at launch time
+        pack_pieces.push_back(pc);
+      }
+      off += count;
+    }
+    ext_size = off;
+    HIPCHK(hipMalloc(&pk_tab, sizeof(int64_t) * std::max<size_t>(host.size(), 1)));
+    if (!host.empty())
+      HIPCHK(hipMemcpy(pk_tab, host.data(), sizeof(int64_t) * host.size(), hipMemcpyHostToDevice));
+  }
+  std::vector<std::string> pack_block;   // full block of each packed piece group
+
+  // full-storage T3 -> ccq's packed layout at dst (T1/T2 copied as they are)
+  void pack(double* dst) {
+    const int64_t head = blk_off.at("t3aaa");
+    HIPCHK(ccqk::launch_copy(stream, dst, tvec.p, head));
+    for (const PackPiece& pc : pack_pieces) {
+      if (pc.image) continue;
+      const double* full = tvec.p + blk_off.at(pack_block[pc.blk]);
+      HIPCHK(ccqk::launch_gather64(stream, pc.rows, pc.cols, full, pk_tab + pc.t_full_row,
+                                   pk_tab + pc.t_full_col, dst + pc.packed_off,
+                                   pk_tab + pc.t_pk_row, pk_tab + pc.t_pk_col, 1.0));
+    }
+  }
+
+  // ccq's packed layout at src -> full storage: every stored value to its
+  // tuple and, with the permutation sign, to all antisymmetric images; Q-space
+  // tuples are zero
+  void unpack(const double* src) {
+    const int64_t head = blk_off.at("t3aaa");
+    HIPCHK(ccqk::launch_copy(stream, tvec.p, src, head));
+    HIPCHK(ccqk::launch_fill(stream, tvec.p + head, t_size - head, 0.0));
+    for (const PackPiece& pc : pack_pieces) {
+      double* full = tvec.p + blk_off.at(pack_block[pc.blk]);
+      HIPCHK(ccqk::launch_gather64(stream, pc.rows, pc.cols, src + pc.packed_off,
+                                   pk_tab + pc.t_pk_row, pk_tab + pc.t_pk_col, full,
+                                   pk_tab + pc.t_full_row, pk_tab + pc.t_full_col, pc.sign));
+    }
   }
 
   // -------------------------------------------------------------------------
@@ -296,6 +476,13 @@ struct ccq_hip_ctx {
           while (std::getline(pss, x, ',')) p[k++] = std::stoi(x);
           cur->upd.perms.insert(cur->upd.perms.end(), p.begin(), p.end());
           cur->upd.signs.push_back(sg);
+        }
+        std::string tok;
+        if (ls >> tok) {
+          if (tok != "mask") throw Fail("bad update record tail " + tok);
+          ls >> cur->upd.vneed >> cur->upd.vforbid;
+          cur->upd.masked = true;
+          if (!active) throw Fail("masked update outside an active-space program");
         }
       } else if (kw == "copy") {
         cur->kind = Section::Copy;
@@ -665,6 +852,14 @@ struct ccq_hip_ctx {
           P.group[h * half + k + 1] = h * 16 + (spins[k] == 'b');
         }
     P.shift = shift;
+    if (u.masked) {
+      P.mask = 1;
+      P.vneed = u.vneed;
+      P.vforbid = u.vforbid;
+      P.occ_start = occ_start;
+      P.vir_end[0] = vir_end[0];
+      P.vir_end[1] = vir_end[1];
+    }
     double* t = tvec.p + blk_off.at(b);
     HIPCHK(ccqk::launch_update(stream, P, t, R.ptr, itab + u.dev_perm_off, dtab + u.dev_sign_off,
                                fdiag_a.p, fdiag_b.p));
@@ -876,9 +1071,25 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (ctx->oa < 0 || ctx->ob < 0 || ctx->va < 0 || ctx->vb < 0 || ctx->nc <= 0)
       throw Fail("inconsistent orbital counts");
     ctx->level = (int)opts->level;
-    if (ctx->level < 2 || ctx->level > 4)
-      throw Fail("level must be 2 (CCSD), 3 (CCSDT) or 4 (CCSDTQ)");
-    ctx->rhf = opts->rhf != 0;
+    if (ctx->level < 2 || ctx->level > 5)
+      throw Fail("level must be 2 (CCSD), 3 (CCSDT), 4 (CCSDTQ) or 5 (CCSDt)");
+    if (ctx->level == 5) {
+      // CCSDt: CCSDT equations with active-space triples, no closed-shell
+      // shortcuts (update_clusters_t3_opt, src/update_cc.F90:325-2191)
+      if (opts->act_occ == -1 && opts->act_unocc == -1)
+        throw Fail("CONFIGURATION ERROR: Active space methods require an active space");
+      ctx->active = true;
+      ctx->level = 3;
+      ctx->act_occ = opts->act_occ;
+      ctx->act_unocc = opts->act_unocc;
+      // src/parser.f90:496-499
+      const int64_t m1 = std::max(ctx->occ_b - ctx->act_occ, ctx->froz);
+      const int64_t m2 = std::min(ctx->occ_a + ctx->act_unocc, ctx->orbs);
+      ctx->occ_start = m1 - ctx->froz;
+      ctx->vir_end[0] = std::max<int64_t>(0, m2 - ctx->occ_a);
+      ctx->vir_end[1] = std::max<int64_t>(0, m2 - ctx->occ_b);
+    }
+    ctx->rhf = opts->rhf != 0 && !ctx->active;
     if (ctx->rhf && ctx->oa != ctx->ob) throw Fail("rhf requires occ_a == occ_b");
     ctx->shift = opts->shift;
     ctx->diis = (int)opts->diis_space;
@@ -898,8 +1109,10 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     ctx->tvec.alloc(ctx->dev_t);
     HIPCHK(hipMemsetAsync(ctx->tvec.p, 0, sizeof(double) * ctx->dev_t, ctx->stream));
     ctx->ring.alloc((int64_t)(ctx->diis + 1) * ctx->vec_size);
+    if (ctx->active) ctx->staging.alloc(ctx->ext_size);
     ctx->partial.alloc(kPartialBlocks * 128);
     if (ctx->level == 2) ctx->load_ir(ctx->rhf ? kIR_CCSD_RHF : kIR_CCSD_UHF);
+    else if (ctx->active) ctx->load_ir(kIR_CCSDT_ACT_UHF);
     else if (ctx->level == 3) ctx->load_ir(ctx->rhf ? kIR_CCSDT_RHF : kIR_CCSDT_UHF);
     else {
 #ifdef CCQ_HAVE_CCSDTQ
@@ -971,20 +1184,33 @@ int ccq_hip_set_ints(ccq_hip_ctx* ctx, const double* f_a, const double* f_b, con
   });
 }
 
-int64_t ccq_hip_t_size(const ccq_hip_ctx* ctx) { return ctx ? ctx->t_size : -1; }
+int64_t ccq_hip_t_size(const ccq_hip_ctx* ctx) { return ctx ? ctx->ext_size : -1; }
 
 int ccq_hip_set_t(ccq_hip_ctx* ctx, const double* t_vec, int64_t t_size) {
   return guarded(ctx, [&] {
-    if (t_size != ctx->t_size) throw Fail("t_size mismatch");
-    HIPCHK(hipMemcpyAsync(ctx->tvec.p, t_vec, sizeof(double) * std::min(ctx->dev_t, ctx->t_size),
-                          hipMemcpyHostToDevice, ctx->stream));
+    if (t_size != ctx->ext_size) throw Fail("t_size mismatch");
+    if (ctx->active) {
+      HIPCHK(hipMemcpyAsync(ctx->staging.p, t_vec, sizeof(double) * ctx->ext_size,
+                            hipMemcpyHostToDevice, ctx->stream));
+      ctx->unpack(ctx->staging.p);
+    } else {
+      HIPCHK(hipMemcpyAsync(ctx->tvec.p, t_vec, sizeof(double) * std::min(ctx->dev_t, ctx->t_size),
+                            hipMemcpyHostToDevice, ctx->stream));
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
   });
 }
 
 int ccq_hip_get_t(ccq_hip_ctx* ctx, double* t_vec, int64_t t_size) {
   return guarded(ctx, [&] {
-    if (t_size != ctx->t_size) throw Fail("t_size mismatch");
+    if (t_size != ctx->ext_size) throw Fail("t_size mismatch");
+    if (ctx->active) {
+      ctx->pack(ctx->staging.p);
+      HIPCHK(hipMemcpyAsync(t_vec, ctx->staging.p, sizeof(double) * ctx->ext_size,
+                            hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      return;
+    }
     HIPCHK(hipMemcpyAsync(t_vec, ctx->tvec.p, sizeof(double) * std::min(ctx->dev_t, ctx->t_size),
                           hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1026,6 +1252,11 @@ int ccq_hip_energy(ccq_hip_ctx* ctx, double* e) {
 int ccq_hip_diis_record(ccq_hip_ctx* ctx, int64_t iter) {
   return guarded(ctx, [&] {
     const int64_t s = ctx->slot_of(iter);
+    if (ctx->active) {
+      // the DIIS vector is ccq's packed t_vec (conv%vec_size = t_size)
+      ctx->pack(ctx->ring.p + s * ctx->vec_size);
+      return;
+    }
     HIPCHK(hipMemcpyAsync(ctx->ring.p + s * ctx->vec_size, ctx->tvec.p,
                           sizeof(double) * ctx->vec_size, hipMemcpyDeviceToDevice, ctx->stream));
   });
@@ -1071,7 +1302,12 @@ int ccq_hip_diis_extrapolate(ccq_hip_ctx* ctx) {
     if (!solve_dense(B, c, m)) throw Fail("calc_diis: DIIS error in DGESV.");
     ccqk::Coeffs cf{};
     for (int k = 0; k < n; ++k) cf.c[k] = c[k];
-    HIPCHK(ccqk::launch_lincomb(ctx->stream, ctx->tvec.p, ctx->ring.p, ctx->vec_size, n, cf));
+    if (ctx->active) {
+      HIPCHK(ccqk::launch_lincomb(ctx->stream, ctx->staging.p, ctx->ring.p, ctx->vec_size, n, cf));
+      ctx->unpack(ctx->staging.p);
+    } else {
+      HIPCHK(ccqk::launch_lincomb(ctx->stream, ctx->tvec.p, ctx->ring.p, ctx->vec_size, n, cf));
+    }
   });
 }
 

@@ -49,6 +49,13 @@ struct UpdateParams {
   int spin[kMaxRank];
   int64_t occ[kMaxRank]; // occupied count of the position's spin (virtual offset)
   double shift;
+  // masked update (active-space triples): only tuples with an occupied index
+  // >= occ_start, an active virtual (index < vir_end[spin]) of a spin in vneed
+  // and no active virtual of a spin in vforbid are updated
+  int mask = 0;
+  int vneed = 0, vforbid = 0;
+  int64_t occ_start = 0;
+  int64_t vir_end[2] = {0, 0};
 };
 
 struct EnergyParams {
@@ -77,10 +84,11 @@ hipError_t launch_gemm(hipStream_t s, const GemmArgs& g);
 hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
                            const int* colA, double* C, const int* rowC, const int* colC,
                            double alpha, int accumulate);
-// Y[rowY[m] + colY[n]] = X[rowX[m] + colX[n]] (int64 tables; view set-up)
+// Y[rowY[m] + colY[n]] = scale * X[rowX[m] + colX[n]] (int64 tables; view
+// set-up, packing of the active-space t_vec)
 hipError_t launch_gather64(hipStream_t s, int64_t M, int64_t N, const double* X,
                            const int64_t* rowX, const int64_t* colX, double* Y,
-                           const int64_t* rowY, const int64_t* colY);
+                           const int64_t* rowY, const int64_t* colY, double scale = 1.0);
 // C (+)= alpha A for identically laid out dense arrays of n doubles (16-byte aligned)
 hipError_t launch_axpy_flat(hipStream_t s, int64_t n, const double* A, double* C, double alpha,
                             int accumulate);

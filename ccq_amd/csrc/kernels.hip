@@ -385,16 +385,18 @@ __global__ __launch_bounds__(256) void perm_add(int64_t M, int N, int tn_shift,
   }
 }
 
-// Y[rowY[m] + colY[n]] = X[rowX[m] + colX[n]] with int64 tables (copies a
-// view out of a base integral array that may exceed 2^31 elements)
+// Y[rowY[m] + colY[n]] = scale * X[rowX[m] + colX[n]] with int64 tables
+// (copies a view out of a base integral array that may exceed 2^31
+// elements; packs / unpacks the active-space t_vec)
 __global__ __launch_bounds__(256) void gather64(int64_t M, int64_t N, const double* __restrict__ X,
                                                 const int64_t* __restrict__ rowX,
                                                 const int64_t* __restrict__ colX,
                                                 double* __restrict__ Y,
                                                 const int64_t* __restrict__ rowY,
-                                                const int64_t* __restrict__ colY) {
+                                                const int64_t* __restrict__ colY, double scale) {
   for (int64_t m = blockIdx.x; m < M; m += gridDim.x)
-    for (int64_t n = threadIdx.x; n < N; n += blockDim.x) Y[rowY[m] + colY[n]] = X[rowX[m] + colX[n]];
+    for (int64_t n = threadIdx.x; n < N; n += blockDim.x)
+      Y[rowY[m] + colY[n]] = scale * X[rowX[m] + colX[n]];
 }
 
 // C (+)= alpha A over identically laid out dense tensors, two doubles per thread
@@ -454,6 +456,19 @@ __global__ void jacobi_update(UpdateParams P, double* __restrict__ t, const doub
     for (int d = 0; d + 1 < rank; ++d)
       if (P.group[d] >= 0 && P.group[d] == P.group[d + 1] && idx[d] >= idx[d + 1]) canon = false;
     if (!canon) continue;
+    if (P.mask) {
+      bool occ_act = false, vir_a = false, vir_b = false;
+      for (int d = 0; d < rank; ++d) {
+        if (!P.isvir[d]) {
+          occ_act |= idx[d] >= P.occ_start;
+        } else if (idx[d] < P.vir_end[P.spin[d]]) {
+          if (P.spin[d]) vir_b = true; else vir_a = true;
+        }
+      }
+      const bool need = ((P.vneed & 1) && vir_a) || ((P.vneed & 2) && vir_b);
+      const bool forbid = ((P.vforbid & 1) && vir_a) || ((P.vforbid & 2) && vir_b);
+      if (!occ_act || !need || forbid) continue;
+    }
     double R = 0.0;
     for (int g = 0; g < P.nimg; ++g) {
       int64_t off = 0;
@@ -823,10 +838,10 @@ hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A,
 
 hipError_t launch_gather64(hipStream_t s, int64_t M, int64_t N, const double* X,
                            const int64_t* rowX, const int64_t* colX, double* Y,
-                           const int64_t* rowY, const int64_t* colY) {
+                           const int64_t* rowY, const int64_t* colY, double scale) {
   if (M <= 0 || N <= 0) return hipSuccess;
   hipLaunchKernelGGL(gather64, dim3((unsigned)std::min<int64_t>(M, 65536)), dim3(256), 0, s, M, N,
-                     X, rowX, colX, Y, rowY, colY);
+                     X, rowX, colX, Y, rowY, colY, scale);
   return hipGetLastError();
 }
 

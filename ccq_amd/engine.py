@@ -20,7 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libccq_hip.so")
 
-LEVELS = {"CCSD": 2, "CCSDT": 3, "CCSDTQ": 4}
+LEVELS = {"CCSD": 2, "CCSDT": 3, "CCSDTQ": 4, "CCSDt": 5}
 
 
 class EngineError(RuntimeError):
@@ -34,7 +34,8 @@ class Dims(ctypes.Structure):
 
 class Opts(ctypes.Structure):
     _fields_ = [("level", ctypes.c_int64), ("rhf", ctypes.c_int64), ("shift", ctypes.c_double),
-                ("diis_space", ctypes.c_int64), ("device", ctypes.c_int64)]
+                ("diis_space", ctypes.c_int64), ("device", ctypes.c_int64),
+                ("act_occ", ctypes.c_int64), ("act_unocc", ctypes.c_int64)]
 
 
 _lib = None
@@ -86,10 +87,12 @@ class Engine:
     """One resident CC problem on one GPU."""
 
     def __init__(self, froz: int, occ_a: int, occ_b: int, orbs: int, calc_type: str = "CCSD",
-                 rhf: bool = False, shift: float = 0.0, diis_space: int = 5, device: int = 0):
+                 rhf: bool = False, shift: float = 0.0, diis_space: int = 5, device: int = 0,
+                 act_occ: int = -1, act_unocc: int = -1):
         self.lib = load_library()
         self.dims = Dims(froz, occ_a, occ_b, orbs)
-        self.opts = Opts(LEVELS[calc_type], int(rhf), float(shift), int(diis_space), int(device))
+        self.opts = Opts(LEVELS[calc_type], int(rhf), float(shift), int(diis_space), int(device),
+                         int(act_occ), int(act_unocc))
         h = ctypes.c_void_p()
         rc = self.lib.ccq_hip_create(ctypes.byref(self.dims), ctypes.byref(self.opts), ctypes.byref(h))
         if rc != 0 or not h.value:

@@ -60,7 +60,23 @@ def _perm_copy_section(name, dst, src, perm_labels):
     return Section(name=name, kind='compute', ops=[op])
 
 
+# update_clusters_t3_opt (src/update_cc.F90:325-2191): T3 residuals in three
+# groups, each formed from the state at the start of the group, with masked
+# updates (block, (vneed, vforbid)): an occupied index and a virtual index of
+# a spin in vneed must be active, no virtual of a spin in vforbid may be.
+CCSDT_ACT_T12 = ['t2ab', 't2aa', 't2bb', 't1a', 't1b']
+CCSDT_ACT_GROUPS = [
+    [('t3aab', (1, 0)), ('t3abb', (2, 0))],     # t3BC_update_opt.f: B1, B4, C1, C4
+    [('t3aaa', (3, 0)), ('t3aab', (2, 1))],     # t3AB_update_opt.f: A, B2, B3
+    [('t3abb', (1, 2)), ('t3bbb', (3, 0))],     # t3CD_update_opt.f: C2, C3, D
+]
+
+
 def build(method: str, rhf: bool):
+    active = method == 'CCSDt'
+    if active:
+        # the active-space path has no closed-shell shortcuts in ccq
+        method, rhf = 'CCSDT', False
     lvl = LEVEL[method]
     if lvl == 2:
         return cp.build(rhf)
@@ -118,6 +134,8 @@ def build(method: str, rhf: bool):
     # sections; that state also serves the next sweep's T2 sections).
     pre = [n for n in order if n.startswith('t2') or n == 't1a']
     post = [n for n in order if n.startswith('t3') or n.startswith('t4')]
+    if active:
+        return prog, _active_sections(prog, res, dress_stm, dressed_needs, pre, post)
     sections = [compile_section(prog, 'dress', dress_stm, dressed_needs(pre),
                                 live_kinds=('dressed',))]
     for name in order:
@@ -149,3 +167,27 @@ def build(method: str, rhf: bool):
                 sections.append(Section(name='copy:t4bbbb', kind='copy',
                                         extra={'dst': 't4bbbb', 'src': 't4aaaa'}))
     return prog, sections
+
+
+def _active_sections(prog, res, dress_stm, dressed_needs, pre, post):
+    """Section list of the CCSDt sweep (T2B, T2A, T2C, T1A, T1B, T3 groups)."""
+    def upd(name, mask=None):
+        rank, nb = NAME_BLOCK[name]
+        extra = {'block': name}
+        if mask is not None:
+            extra['mask'] = mask
+        return Section(name='update:' + name, kind='update', target=res_key(rank, nb), extra=extra)
+
+    sections = [compile_section(prog, 'dress', dress_stm, dressed_needs(pre),
+                                live_kinds=('dressed',))]
+    for name in CCSDT_ACT_T12:
+        if name == 't1b':
+            sections.append(compile_section(prog, 'dress:t1b', dress_stm, dressed_needs(['t1b']),
+                                            live_kinds=('dressed',)))
+        sections += [res[name], upd(name)]
+    sections.append(compile_section(prog, 'dress:t3', dress_stm, dressed_needs(post),
+                                    live_kinds=('dressed',)))
+    for group in CCSDT_ACT_GROUPS:
+        sections += [res[name] for name, _ in group]
+        sections += [upd(name, mask) for name, mask in group]
+    return sections

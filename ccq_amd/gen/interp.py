@@ -66,10 +66,13 @@ def block_shape(key: BlockKey, dims: Dims):
 
 class Interpreter:
     def __init__(self, prog, sections, dims: Dims, base: dict, amps: dict,
-                 shift: float = 0.0):
+                 shift: float = 0.0, active=None):
         self.prog, self.sections, self.dims = prog, sections, dims
         self.base, self.amps = base, amps
         self.shift = shift
+        # active space of masked updates: (first active occupied index,
+        # active alpha virtuals, active beta virtuals)
+        self.active = active
         self.store: dict[BlockKey, np.ndarray] = {}
 
     def get(self, key: BlockKey):
@@ -101,7 +104,7 @@ class Interpreter:
                     self.store[op.out] += val
                 del terms
         elif sec.kind == 'update':
-            self.update(sec.extra['block'], sec.target)
+            self.update(sec.extra['block'], sec.target, sec.extra.get('mask'))
         elif sec.kind == 'copy':
             self.amps[sec.extra['dst']][...] = self.amps[sec.extra['src']]
 
@@ -114,10 +117,37 @@ class Interpreter:
             R += sign * U.transpose(perm)
         return R / len(imgs)
 
-    def update(self, block: str, key: BlockKey):
+    def update(self, block: str, key: BlockKey, mask=None):
         R = self.residual(block, key)
         D = self.denominators(key)
-        self.amps[block] -= R / D
+        if mask is None:
+            self.amps[block] -= R / D
+            return
+        m = self.mask(key, *mask)
+        self.amps[block][m] -= R[m] / D[m]
+
+    def mask(self, key: BlockKey, vneed: int, vforbid: int):
+        """Tuples with an active occupied index, an active virtual of a spin in
+        vneed and no active virtual of a spin in vforbid."""
+        occ_start, vend_a, vend_b = self.active
+        shape = block_shape(key, self.dims)
+        idx = np.indices(shape)
+        occ = np.zeros(shape, bool)
+        vir = [np.zeros(shape, bool), np.zeros(shape, bool)]
+        for p, (c, s) in enumerate(zip(key.spaces, key.spins)):
+            if c == 'o':
+                occ |= idx[p] >= occ_start
+            else:
+                vir[s] = vir[s] | (idx[p] < (vend_b if s else vend_a))
+        m = np.zeros(shape, bool)
+        for s in (0, 1):
+            if vneed >> s & 1:
+                m |= vir[s]
+        m &= occ
+        for s in (0, 1):
+            if vforbid >> s & 1:
+                m &= ~vir[s]
+        return m
 
     def denominators(self, key: BlockKey):
         fa = np.diag(self.base['Fa'])

@@ -17,7 +17,7 @@ program ccq_drv
     integer, parameter :: dp = c_double
     character(len=512) :: inp, line, key, val, onebody, twobody, calc_type
     integer(c_int64_t) :: froz_spin, nel, nvirt, mult, froz, occ_a, occ_b, orbs
-    integer(c_int64_t) :: diis_space, max_iter, level, rhf_flag
+    integer(c_int64_t) :: diis_space, max_iter, level, rhf_flag, act_occ, act_unocc
     real(dp) :: tol, shift
     real(dp), allocatable :: e1int(:,:), e2int(:,:,:,:), f_a(:,:), f_b(:,:)
     real(dp), allocatable :: v_aa(:,:,:,:), v_ab(:,:,:,:)
@@ -36,7 +36,7 @@ program ccq_drv
     end if
 
     ! defaults (src/parser.f90:779-844)
-    froz_spin = 0; nel = -1; nvirt = -1; mult = 1
+    froz_spin = 0; nel = -1; nvirt = -1; mult = 1; act_occ = -1; act_unocc = -1
     tol = 1.0e-4_dp; diis_space = 5; max_iter = 60; shift = 0.0_dp; rhf = .false.
     onebody = 'onebody.inp'; twobody = 'twobody.inp'; calc_type = 'CCSD'
 
@@ -69,6 +69,8 @@ program ccq_drv
         case ('onebody'); onebody = val
         case ('twobody'); twobody = val
         case ('calc_type'); calc_type = val
+        case ('act_occ'); read(val, *) act_occ
+        case ('act_unocc'); read(val, *) act_unocc
         end select
     end do
     close(unit_in)
@@ -82,6 +84,13 @@ program ccq_drv
     case ('CCSD'); level = 2
     case ('CCSDT'); level = 3
     case ('CCSDTQ'); level = 4
+    case ('CCSDt')
+        level = 5
+        ! src/parser.f90:251-254
+        if (act_occ == -1 .and. act_unocc == -1) then
+            write(0, '(a)') 'CONFIGURATION ERROR: Active space methods require an active space'
+            call exit(1)
+        end if
     case default
         write(0, '(a)') 'ccq_drv: calc_type '//trim(calc_type)//' is not supported'
         call exit(1)
@@ -165,7 +174,7 @@ program ccq_drv
 
     dims = ccq_hip_dims(froz, occ_a, occ_b, orbs)
     rhf_flag = merge(1_c_int64_t, 0_c_int64_t, rhf)
-    opts = ccq_hip_opts(level, rhf_flag, shift, diis_space, 0_c_int64_t)
+    opts = ccq_hip_opts(level, rhf_flag, shift, diis_space, 0_c_int64_t, act_occ, act_unocc)
     if (ccq_hip_create(dims, opts, ctx) /= 0) then
         write(0, '(a)') 'ERROR in ccq_hip_create'
         call exit(1)

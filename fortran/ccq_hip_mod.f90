@@ -19,6 +19,7 @@ module ccq_hip_mod
         integer(c_int64_t) :: level, rhf
         real(c_double) :: shift
         integer(c_int64_t) :: diis_space, device
+        integer(c_int64_t) :: act_occ, act_unocc
     end type ccq_hip_opts
 
     interface

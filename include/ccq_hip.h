@@ -43,11 +43,17 @@ typedef struct {
 
 /* run_t options used by the iteration (src/system.f90:108-170). */
 typedef struct {
-  int64_t level;       /* 2 CCSD, 3 CCSDT, 4 CCSDTQ (run%lvl_t, run%lvl_q) */
-  int64_t rhf;         /* run%rhf: closed-shell copies T2C=T2A, T1B=T1A ... */
+  int64_t level;       /* 2 CCSD, 3 CCSDT, 4 CCSDTQ (run%lvl_t, run%lvl_q),
+                          5 CCSDt: active-space triples (calc_type=CCSDt,
+                          run%act_ind_t = 1, src/parser.f90:380-390)       */
+  int64_t rhf;         /* run%rhf: closed-shell copies T2C=T2A, T1B=T1A ...
+                          (ignored by CCSDt, as update_clusters_t3_opt does) */
   double shift;        /* run%shift: denominator shift                     */
   int64_t diis_space;  /* run%diis_space (<= 12)                           */
   int64_t device;      /* HIP device ordinal                               */
+  int64_t act_occ;     /* sys%act_occ, sys%act_unocc (CCSDt only; -1 unset;
+                          src/parser.f90:496-499)                          */
+  int64_t act_unocc;
 } ccq_hip_opts;
 
 /* Create a context: allocate the resident tensors and plan the program.
@@ -59,7 +65,9 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
 int ccq_hip_set_ints(ccq_hip_ctx* ctx, const double* f_a, const double* f_b,
                      const double* v_aa, const double* v_ab, const double* v_bb);
 
-/* Number of doubles in ccq's t_vec (cc%t_size = pos(10)-1). */
+/* Number of doubles in ccq's t_vec: cc%t_size = pos(10)-1 (get_t_sizes,
+ * src/utils/cc_utils.f90:479-559), or pos(16)-1 with the ten P-space T3
+ * blocks of CCSDt (get_t_sizes_act, :560-631). */
 int64_t ccq_hip_t_size(const ccq_hip_ctx* ctx);
 
 /* Copy cc%t_vec to / from the device (T1..T3; T4 stays resident). */

@@ -34,8 +34,9 @@ def test_library_exports_every_declared_symbol():
 def test_embedded_ir_is_current():
     gh = emit.generator_hash()
     for method in emit.METHODS:
-        for tag in ("uhf", "rhf"):
-            path = os.path.join(ROOT, "ccq_amd", "csrc", f"ir_{method.lower()}_{tag}.inc")
+        for rhf in emit.VARIANTS[method]:
+            tag = "rhf" if rhf else "uhf"
+            path = os.path.join(ROOT, "ccq_amd", "csrc", f"ir_{emit.STEM[method]}_{tag}.inc")
             with open(path) as f:
                 assert f"generator {gh}" in f.readline(), path
     for rhf in (False, True):
