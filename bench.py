@@ -12,9 +12,11 @@ Protocol
      residuum, DIIS every 6th), synchronising the stream around the timed
      region and after every iteration (the energy is read back each time);
   4. value = wall time until the first iteration with |E - E_conv| < 1e-7.
-  Roofline: the dominant kernel (the o^2 v^4 ladder GEMM, M=v^2 N=o^2 K=v^2)
-  timed with HIP events on the engine's stream during one extra profiled
-  iteration; algorithmic FLOPs 2*M*N*K per launch.
+  Roofline: the dominant GEMM class (the contraction shape with the largest
+  share of the sweep) timed with HIP events on the engine's stream during one
+  extra profiled iteration; algorithmic FLOPs 2*M*N*K per launch.
+  traffic: HBM bytes per launch of that class from the rocprofv3 FETCH_SIZE /
+  WRITE_SIZE runs summarised under profiles/ (copy-calibrated), when measured.
   cpu_baseline: the numpy oracle (same equations, same sweep, OpenBLAS) on a
   bounded sample of the same workload, extrapolated to the same iteration count.
 
@@ -47,8 +49,8 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_workload(cfg_idx: int):
-    s = synth.config_system(cfg_idx)
+def make_workload(name: str):
+    s = synth.workload_system(name)
     ints = build_integrals(s.h, s.e2int(), s.enuc, 0, s.nocc, s.nocc)
     return s, ints
 
@@ -112,22 +114,56 @@ def cpu_baseline(s, ints, n_iter_needed, budget_s=25.0):
                        f"({per_iter:.2f} s each, numpy/OpenBLAS), x {n_iter_needed} iterations")
 
 
+def measured_traffic(kernel_class: str):
+    """HBM bytes per launch of a GEMM class, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised under profiles/ (None if not measured)."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and \
+                        parts[1] == kernel_class:
+                    return float(parts[2])
+    return None
+
+
+def copy_probe(nbytes: int):
+    """A plain device-to-device copy of a known size, three times: the
+    reference reading for FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {nbytes} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--workload", default="ccsd-n100", choices=sorted(
+        k for k, v in synth.WORKLOADS.items() if v["method"] == "CCSD"))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rhf", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (calibrates HBM counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design "
-                                   "only (see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this engine runs on one GPU; sharding over several GPUs "
+                                   "is designed but not built, so --gpus must be 1"}))
         sys.exit(2)
     diis = 5
     t0 = time.perf_counter()
-    s, ints = make_workload(args.config)
+    s, ints = make_workload(args.workload)
     o, n = s.nocc, s.norb
     log(f"workload built in {time.perf_counter() - t0:.1f}s: n={n} o={o} v={n - o}")
     eng = E.Engine(0, o, o, n, "CCSD", rhf=args.rhf, diis_space=diis)
@@ -192,14 +228,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded integrals, ccq onebody/twobody construction)",
-        "config": {"workload": f"configs[{args.config}] CCSD plain FP64, n={n} o={o} v={n - o}, "
-                               f"L={synth.CONFIGS[args.config]['nL']}, seed "
-                               f"{synth.CONFIGS[args.config]['seed']}, diis 5, "
+        "config": {"workload": f"{args.workload}: CCSD plain FP64, n={n} o={o} v={n - o}, "
+                               f"L={synth.WORKLOADS[args.workload]['nL']}, seed "
+                               f"{synth.WORKLOADS[args.workload]['seed']}, diis 5, "
                                f"{'rhf' if args.rhf else 'uhf'} sweep",
                    "iterations_to_1e-7": n_needed, "e_conv": e_conv,
                    "mode": "plain FP64 (mixed precision and learned guess not built yet)"},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
+                     "traffic": measured_traffic(dom_name),
                      "kernel": dom_name, "avg_launch_us": avg_s * 1e6,
                      "flops_per_launch": fl_per,
                      "all_gemm_tflops": gemm_fl / (gemm_ms * 1e-3) / 1e12},

@@ -101,17 +101,19 @@ def regime_sigma(norb: int, nocc: int, nL: int) -> float:
 
 
 # The workloads this engine is measured on (spatial orbitals, occupied, L, seed).
-CONFIGS = {
-    0: dict(norb=24, nocc=5, nL=48, seed=0, method="CCSD"),
-    1: dict(norb=100, nocc=10, nL=200, seed=1, method="CCSD", sigma=regime_sigma(100, 10, 200)),
-    2: dict(norb=40, nocc=5, nL=80, seed=2, method="CCSDT"),
-    3: dict(norb=80, nocc=7, nL=48, seed=3, method="CCSDt", d2h_irreps=True),
-    4: dict(norb=20, nocc=4, nL=40, seed=4, method="CCSDTQ"),
+# Named synthetic workloads (sizes n/o, auxiliary vectors L, seed, method).
+WORKLOADS = {
+    "ccsd-n24": dict(norb=24, nocc=5, nL=48, seed=0, method="CCSD"),
+    "ccsd-n100": dict(norb=100, nocc=10, nL=200, seed=1, method="CCSD",
+                      sigma=regime_sigma(100, 10, 200)),
+    "ccsdt-n40": dict(norb=40, nocc=5, nL=80, seed=2, method="CCSDT"),
+    "ccsdt-act-n80": dict(norb=80, nocc=7, nL=48, seed=3, method="CCSDt", d2h_irreps=True),
+    "ccsdtq-n20": dict(norb=20, nocc=4, nL=40, seed=4, method="CCSDTQ"),
 }
 
 
-def config_system(idx: int) -> System:
-    c = dict(CONFIGS[idx])
+def workload_system(name: str) -> System:
+    c = dict(WORKLOADS[name])
     c.pop("method")
     return make_system(**c)
 

@@ -49,8 +49,8 @@ def test_one_sweep_matches_oracle(n, oa, ob, rhf):
     eng.close()
 
 
-def test_full_ccsd_config0_parity():
-    """configs[0]: 24 orbitals, 5 occupied, tol 1e-9, diis 5: energies and T at convergence."""
+def test_full_ccsd_n24_parity():
+    """24 orbitals, 5 occupied, tol 1e-9, diis 5: energies and T at convergence."""
     _say("case config0 full solve")
     c = Case(24, 5, 48, 0)
     opts = osolver.Options(calc_type='CCSD', tol=1e-9, max_iter=60, diis_space=5)
