@@ -34,6 +34,7 @@ struct GemmArgs {
   int rm = 1, rn = 1, wk = 1;         // register-direct wave tile / waves along K
   int nsplit = 1;
   int64_t kchunk = 0;
+  int group = 1;           // tile rows per group of the XCD-aware tile order
   // split-K workspace of at least gemm_ws_need doubles
   double* ws = nullptr;
 };

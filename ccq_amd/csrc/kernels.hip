@@ -58,11 +58,35 @@ __device__ __forceinline__ double sum_strided(const double* __restrict__ q, int6
   return (a0 + a1) + (a2 + a3);
 }
 
+// Tile order.  Workgroups are dispatched round-robin over the 8 XCDs of
+// MI355X, each with its own L2, in linear id order (x fastest, then y, z).
+// The linear id is remapped so that every XCD receives a contiguous range of
+// logical tiles, and logical tiles are ordered in groups of `group` tile rows
+// (all of one split-K slice first), so the A row panels and B column panels a
+// workgroup needs are mostly shared with the other workgroups on its XCD.
+__device__ __forceinline__ void tile_coords(int group, int& tx, int& ty, int& tz) {
+  const int tm = gridDim.x, tn = gridDim.y;
+  const int total = tm * tn * (int)gridDim.z;
+  const int lin = blockIdx.x + tm * (blockIdx.y + tn * blockIdx.z);
+  const int q = total >> 3, r = total & 7;
+  const int xcd = lin & 7, slot = lin >> 3;
+  const int logical = xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+  const int per_z = tm * tn;
+  tz = logical / per_z;
+  const int pid = logical - tz * per_z;
+  const int gsize = group * tn;
+  const int first = (pid / gsize) * group;
+  const int gm = tm - first < group ? tm - first : group;
+  const int in = pid - (pid / gsize) * gsize;
+  tx = first + in % gm;
+  ty = in / gm;
+}
+
 // Split-K: workgroup z of a tile stores its partial product of K range
 // [z*kchunk, (z+1)*kchunk) at ws[z][m][n] (M x N, row-major); gemm_splitk_reduce
 // sums the nsplit partials of every element in a fixed order and writes C.
-__device__ __forceinline__ void store_partial(const GemmArgs& g, int64_t m, int64_t n, double v) {
-  g.ws[((int64_t)blockIdx.z * g.M + m) * g.N + n] = v;
+__device__ __forceinline__ void store_partial(const GemmArgs& g, int z, int64_t m, int64_t n, double v) {
+  g.ws[((int64_t)z * g.M + m) * g.N + n] = v;
 }
 
 // LDS tile of an operand: R x BK values.  KF (k-fastest in global memory):
@@ -126,7 +150,7 @@ struct Loader {
 // registers while the current one is multiplied out of LDS.  blockIdx.z owns
 // K range [z*kchunk, (z+1)*kchunk) (split-K, completed by gemm_splitk_reduce).
 template <int BM, int BN, int BK, bool AKF, bool BKF>
-__global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g) {
+__global__ __launch_bounds__(256, (BM * BN >= 128 * 128) ? 1 : 2) void gemm_f64_tab(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   using TA = LdsTile<BM, AKF, BK>;
@@ -136,9 +160,11 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int64_t m0 = (int64_t)blockIdx.x * BM;
-  const int64_t n0 = (int64_t)blockIdx.y * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  int tx, ty, tz;
+  tile_coords(g.group, tx, ty, tz);
+  const int64_t m0 = (int64_t)tx * BM;
+  const int64_t n0 = (int64_t)ty * BN;
+  const int64_t kbeg = (int64_t)tz * g.kchunk;
   const int64_t kend = (kbeg + g.kchunk < g.K) ? kbeg + g.kchunk : g.K;
   const int wm = (wave & 1) * WM;
   const int wn = (wave >> 1) * WN;
@@ -199,7 +225,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f64_tab(GemmArgs g) {
         const int64_t gm = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
         if (gm >= g.M) continue;
         if (g.nsplit == 1) store_c(g, gm, gn, acc[i][j][v]);
-        else store_partial(g, gm, gn, acc[i][j][v]);
+        else store_partial(g, tz, gm, gn, acc[i][j][v]);
       }
   }
 }
@@ -256,9 +282,11 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wk = wave % WK, wmi = wave / WK;
   const int r = lane & 15, q = lane >> 4;
-  const int64_t m0 = (int64_t)blockIdx.x * TBM, n0 = (int64_t)blockIdx.y * TBN;
+  int tx, ty, tz;
+  tile_coords(g.group, tx, ty, tz);
+  const int64_t m0 = (int64_t)tx * TBM, n0 = (int64_t)ty * TBN;
   const int64_t wm0 = m0 + wmi * 16 * RM;
-  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  const int64_t kbeg = (int64_t)tz * g.kchunk;
   const int64_t kend = (kbeg + g.kchunk < g.K) ? kbeg + g.kchunk : g.K;
   int ra[RM], cb[RN];
 #pragma unroll
@@ -329,7 +357,7 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
         const int64_t gm = wm0 + 16 * i + q + 4 * v;
         if (gm >= g.M) continue;
         if (g.nsplit == 1) store_c(g, gm, gn, acc[i][j][v]);
-        else store_partial(g, gm, gn, acc[i][j][v]);
+        else store_partial(g, tz, gm, gn, acc[i][j][v]);
       }
   }
 }
@@ -337,26 +365,27 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
 // C (+)= alpha * sum_z ws[z] for split-K GEMMs.  P = 2^pshift threads share an
 // element (z = p, p + P, ...), combined through LDS in p order: the summation
 // order depends only on nsplit, so results are deterministic.
+// Elements are addressed as (m = blockIdx.y, n = blockIdx.x * EPB + e), so no
+// integer division is needed.
 __global__ __launch_bounds__(256) void gemm_splitk_reduce(GemmArgs g, int pshift) {
   __shared__ double red[256];
   const int P = 1 << pshift;
   const int64_t total = g.M * g.N;
   const int p = threadIdx.x & (P - 1);
-  const int64_t e = (int64_t)blockIdx.x * (256 >> pshift) + (threadIdx.x >> pshift);
-  double s = 0.0;
-  if (e < total) {
-    const int cnt = (g.nsplit - p + P - 1) / P;
-    s = sum_strided(g.ws + (int64_t)p * total + e, (int64_t)P * total, cnt);
-  }
-  if (P > 1) {
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (p != 0) return;
-    for (int x = 1; x < P; ++x) s += red[threadIdx.x + x];
-  }
-  if (e < total) {
-    const int64_t m = e / g.N, n = e - m * g.N;
-    store_c(g, m, n, s);
+  const int64_t n = (int64_t)blockIdx.x * (256 >> pshift) + (threadIdx.x >> pshift);
+  const bool in = n < g.N;
+  const int cnt = (g.nsplit - p + P - 1) / P;
+  for (int64_t m = blockIdx.y; m < g.M; m += gridDim.y) {
+    double s = 0.0;
+    if (in) s = sum_strided(g.ws + (int64_t)p * total + m * g.N + n, (int64_t)P * total, cnt);
+    if (P > 1) {
+      red[threadIdx.x] = s;
+      __syncthreads();
+      if (p == 0)
+        for (int x = 1; x < P; ++x) s += red[threadIdx.x + x];
+      __syncthreads();
+    }
+    if (in && p == 0) store_c(g, m, n, s);
   }
 }
 
@@ -652,7 +681,7 @@ int64_t gemm_ws_need(const GemmArgs& g) { return g.nsplit > 1 ? g.nsplit * g.M *
 static double splitk_cycles(int64_t M, int64_t N, int64_t split) {
   if (split <= 1) return 0.0;
   const double hbm = 8.0e12 / 2.4e9;
-  return 2.0 * split * M * N * 8.0 / hbm + 5000.0;
+  return 2.0 * split * M * N * 8.0 / hbm + 8000.0;
 }
 
 void gemm_plan(GemmArgs& g, int64_t ws_cap) {
@@ -683,11 +712,11 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
   // WG cannot hide the global-load latency of a slab (~1500 cycles); every
   // slab costs two barriers and the LDS round trip (~300 cycles).
   struct Shape { int bm, bn, occ; };
-  const Shape shapes[4] = {{128, 128, 2}, {128, 64, 3}, {64, 128, 3}, {64, 64, 4}};
+  // resident WGs per CU (128x128 uses more than 256 VGPRs: one)
+  const Shape shapes[4] = {{128, 128, 1}, {128, 64, 3}, {64, 128, 3}, {64, 64, 4}};
   for (int bk : {16, 32}) {
     const int64_t ksteps = cdiv(g.K, bk);
     for (const Shape& sh : shapes) {
-      if (bk == 32 && sh.bm == 128 && sh.bn == 128) continue;   // exceeds 256 VGPRs
       const int64_t tiles = cdiv(g.M, sh.bm) * cdiv(g.N, sh.bn);
       const double slab = (sh.bm / 32.0) * (sh.bn / 32.0) * (bk / 4.0) * 64.0;
       const int64_t maxsplit = std::max<int64_t>(1, std::min<int64_t>(ksteps / 2, 512));
@@ -733,6 +762,12 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
       }
     }
   }
+  // tile-row group of the XCD-aware order: about square per-XCD blocks
+  int64_t tbm, tbn;
+  tile_dims(g, tbm, tbn);
+  const int64_t tm = cdiv(g.trans ? g.N : g.M, tbm);
+  const double per_xcd = (double)gemm_tiles(g) * g.nsplit / 8.0;
+  g.group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, (int64_t)std::lround(std::sqrt(per_xcd))));
 }
 
 template <int BM, int BN, int KS>
@@ -794,7 +829,7 @@ static hipError_t launch_kernels(hipStream_t s, const GemmArgs& g) {
   }
   dim3 grid((unsigned)cdiv(g.M, g.bm), (unsigned)cdiv(g.N, g.bn), (unsigned)g.nsplit);
   if (g.bk == 32) {
-    if (g.bm == 128 && g.bn == 128) return hipErrorInvalidValue;
+    if (g.bm == 128 && g.bn == 128) launch_tile<128, 128, 32>(s, grid, g);
     else if (g.bm == 128) launch_tile<128, 64, 32>(s, grid, g);
     else if (g.bn == 128) launch_tile<64, 128, 32>(s, grid, g);
     else launch_tile<64, 64, 32>(s, grid, g);
@@ -818,8 +853,9 @@ hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
   while ((1 << pshift) < g.nsplit && (1 << pshift) < 32 && g.M * g.N * (2 << pshift) <= 262144)
     ++pshift;
   const int64_t per_block = 256 >> pshift;
-  hipLaunchKernelGGL(gemm_splitk_reduce, dim3((unsigned)cdiv(g.M * g.N, per_block)), dim3(256), 0,
-                     s, g, pshift);
+  hipLaunchKernelGGL(gemm_splitk_reduce,
+                     dim3((unsigned)cdiv(g.N, per_block), (unsigned)std::min<int64_t>(g.M, 65535)),
+                     dim3(256), 0, s, g, pshift);
   return hipGetLastError();
 }
 
