@@ -27,6 +27,7 @@
 #include "ir_ccsdt_rhf.inc"
 #include "ir_ccsdt_uhf.inc"
 #include "ir_ccsdt_act_uhf.inc"
+#include "ir_cct3_uhf.inc"
 #if __has_include("ir_ccsdtq_uhf.inc") && __has_include("ir_ccsdtq_rhf.inc")
 #include "ir_ccsdtq_rhf.inc"
 #include "ir_ccsdtq_uhf.inc"
@@ -108,11 +109,15 @@ struct UpdateStep {
   int dev_sign_off = 0;
   bool masked = false;       // active-space triples: update only the masked tuples
   int vneed = 0, vforbid = 0;
+  bool left = false;         // left-CC step: H-bar denominators, |residual|^2 partials
+  int sq_off = 0, sq_n = 0;  // this step's slice of the partial-sum buffer
+  int m3 = -1;               // triples correction: the M3 accumulator (resid = L3)
 };
 
 struct Section {
   std::string name;
-  enum { Compute, Update, Copy } kind = Compute;
+  std::string prog = "sweep";   // program the section belongs to
+  enum { Compute, Update, Copy, Cct3 } kind = Compute;
   std::vector<Op> ops;
   UpdateStep upd;
   std::string dst, src;
@@ -177,20 +182,38 @@ struct ccq_hip_ctx {
   std::map<std::string, int> tindex;
   std::vector<Section> sections;
   bool ints_ready = false;
+  // CC(t;3) tail (level 6): H-bar, left-CCSD and the triples correction run
+  // as programs of a second IR on the same resident tensors
+  bool tail = false;
+  DevBuf lvec;               // ccq's l_vec: T1/T2 layout
+  int64_t l_size = 0;
+  double left_ecor = 0.0;    // E_cor in the left denominators
+  bool left_mode = false;    // DIIS entry points act on l_vec
+  DevBuf lsq, cpart;         // left |LH|^2 partials; correction partials
+  int64_t lsq_n = 0;
+  int64_t* d3tab = nullptr;  // cal_d3 gather tables
+  struct Had { int out; const double* X; const double* Y; int64_t M, K, xr, xk, yr, yk; double scale; };
+  std::vector<Had> hads;
   // profiling
   bool profile = false;
   std::map<std::string, ProfEntry> prof;
   std::vector<double> host_partial;
 
   ~ccq_hip_ctx() {
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& kv : graphs) drop_graph(kv.second);
+    if (d3tab) (void)hipFree(d3tab);
     if (tables32) (void)hipFree(tables32);
     if (pk_tab) (void)hipFree(pk_tab);
     if (gather_tab) (void)hipFree(gather_tab);
     if (itab) (void)hipFree(itab);
     if (dtab) (void)hipFree(dtab);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // l1a, l3aab, m3abb ... have the shape of t1a, t3aab, t3abb
+  static std::string dimkey(const std::string& base) {
+    if (!base.empty() && (base[0] == 'l' || base[0] == 'm')) return "t" + base.substr(1);
+    return base;
   }
 
   int64_t dimsym(const std::string& s) const {
@@ -400,16 +423,48 @@ struct ccq_hip_ctx {
   // -------------------------------------------------------------------------
   // IR loading
 
-  void load_ir(const char* text) {
+  static void parse_images(std::istringstream& ls, UpdateStep& u) {
+    for (int g = 0; g < u.nimg; ++g) {
+      std::string tok;
+      ls >> tok;
+      auto colon = tok.find(':');
+      if (colon == std::string::npos) throw Fail("bad image token " + tok);
+      std::string ps = tok.substr(0, colon);
+      double sg = std::stod(tok.substr(colon + 1));
+      std::vector<int> p(8, 0);
+      std::istringstream pss(ps);
+      std::string x;
+      int k = 0;
+      while (std::getline(pss, x, ',') && k < 8) p[k++] = std::stoi(x);
+      u.perms.insert(u.perms.end(), p.begin(), p.end());
+      u.signs.push_back(sg);
+    }
+  }
+
+  // Load one IR.  Intermediates of a second IR get the prefix ns so that the
+  // programs never share scratch; views and amplitude blocks with the same
+  // id are the same tensor.
+  void load_ir(const char* text, const std::string& ns = "") {
     std::istringstream in(text);
     std::string line;
     Section* cur = nullptr;
+    std::string prog = "sweep";
+    std::map<std::string, int> local;
+    auto tid = [&](const std::string& n) {
+      auto it = local.find(n);
+      if (it == local.end()) throw Fail("IR names undeclared tensor " + n);
+      return it->second;
+    };
     while (std::getline(in, line)) {
       if (line.empty() || line[0] == '#') continue;
       std::istringstream ls(line);
       std::string kw;
       ls >> kw;
       if (kw == "ccq-ir" || kw == "end") continue;
+      if (kw == "program") {
+        ls >> prog;
+        continue;
+      }
       if (kw == "tensor") {
         Tensor t;
         std::string kind;
@@ -429,30 +484,44 @@ struct ccq_hip_ctx {
         } else if (kind == "amp" || kind == "resid") {
           t.kind = kind == "amp" ? TKind::Amp : TKind::Resid;
           ls >> t.base;
-          auto& d = blk_dims.at(t.base);
+          auto& d = blk_dims.at(dimkey(t.base));
           t.rank = (int)d.size();
           for (int i = 0; i < t.rank; ++i) t.dims[i] = d[i];
         } else {
           throw Fail("bad tensor kind " + kind);
         }
-        tindex[t.id] = (int)tensors.size();
-        tensors.push_back(t);
+        const std::string name = t.id;
+        if (t.kind == TKind::Inter) t.id = ns + t.id;
+        auto hit = tindex.find(t.id);
+        if (hit != tindex.end()) {
+          const Tensor& o = tensors[hit->second];
+          bool same = o.kind == t.kind && o.base == t.base && o.rank == t.rank &&
+                      t.kind != TKind::Inter && t.kind != TKind::Resid;
+          for (int i = 0; same && i < t.rank; ++i) same = o.spec[i] == t.spec[i];
+          if (!same) throw Fail("IR redeclares tensor " + t.id);
+          local[name] = hit->second;
+        } else {
+          tindex[t.id] = (int)tensors.size();
+          local[name] = (int)tensors.size();
+          tensors.push_back(t);
+        }
       } else if (kw == "section") {
         sections.emplace_back();
         cur = &sections.back();
         ls >> cur->name;
+        cur->prog = prog;
       } else if (kw == "op") {
         Op op;
         std::string out, a, b;
         int assign, nin;
         ls >> out >> assign >> op.coef >> nin;
-        op.out = tindex.at(out);
+        op.out = tid(out);
         op.assign = assign != 0;
         ls >> a >> op.la;
-        op.a = tindex.at(a);
+        op.a = tid(a);
         if (nin == 2) {
           ls >> b >> op.lb;
-          op.b = tindex.at(b);
+          op.b = tid(b);
           op.gemm = true;
         }
         ls >> op.lc;
@@ -462,28 +531,27 @@ struct ccq_hip_ctx {
         cur->kind = Section::Update;
         std::string rid;
         ls >> cur->upd.block >> rid >> cur->upd.nimg;
-        cur->upd.resid = tindex.at(rid);
-        for (int g = 0; g < cur->upd.nimg; ++g) {
-          std::string tok;
-          ls >> tok;
-          auto colon = tok.find(':');
-          std::string ps = tok.substr(0, colon);
-          double sg = std::stod(tok.substr(colon + 1));
-          std::vector<int> p(8, 0);
-          std::istringstream pss(ps);
-          std::string x;
-          int k = 0;
-          while (std::getline(pss, x, ',')) p[k++] = std::stoi(x);
-          cur->upd.perms.insert(cur->upd.perms.end(), p.begin(), p.end());
-          cur->upd.signs.push_back(sg);
-        }
+        cur->upd.resid = tid(rid);
+        parse_images(ls, cur->upd);
         std::string tok;
-        if (ls >> tok) {
-          if (tok != "mask") throw Fail("bad update record tail " + tok);
-          ls >> cur->upd.vneed >> cur->upd.vforbid;
-          cur->upd.masked = true;
-          if (!active) throw Fail("masked update outside an active-space program");
+        while (ls >> tok) {
+          if (tok == "mask") {
+            ls >> cur->upd.vneed >> cur->upd.vforbid;
+            cur->upd.masked = true;
+            if (!active) throw Fail("masked update outside an active-space program");
+          } else if (tok == "left") {
+            cur->upd.left = true;
+          } else {
+            throw Fail("bad update record tail " + tok);
+          }
         }
+      } else if (kw == "cct3") {
+        cur->kind = Section::Cct3;
+        std::string l3, m3;
+        ls >> cur->upd.block >> l3 >> m3 >> cur->upd.nimg;
+        cur->upd.resid = tid(l3);
+        cur->upd.m3 = tid(m3);
+        parse_images(ls, cur->upd);
       } else if (kw == "copy") {
         cur->kind = Section::Copy;
         ls >> cur->dst >> cur->src;
@@ -513,7 +581,11 @@ struct ccq_hip_ctx {
           t.strides[i] = s;
           s *= t.dims[i];
         }
-        if (t.kind == TKind::Amp) {
+        if (t.kind == TKind::Amp && t.base[0] == 'l') {
+          const std::string k = dimkey(t.base);
+          if (!tail || blk_off.at(k) + s > l_size) throw Fail("left amplitude block outside l_vec");
+          t.ptr = lvec.p + blk_off.at(k);
+        } else if (t.kind == TKind::Amp) {
           if (blk_off.at(t.base) + s > dev_t) throw Fail("amplitude block beyond device t_vec");
           t.ptr = tvec.p + blk_off.at(t.base);
         } else {
@@ -684,14 +756,21 @@ struct ccq_hip_ctx {
     };
     std::vector<int> ih;
     std::vector<double> dh;
+    lsq_n = 0;
     for (auto& sec : sections) {
-      if (sec.kind == Section::Update) {
+      if (sec.kind == Section::Update || sec.kind == Section::Cct3) {
         sec.upd.dev_perm_off = (int)ih.size();
         ih.insert(ih.end(), sec.upd.perms.begin(), sec.upd.perms.end());
         sec.upd.dev_sign_off = (int)dh.size();
         dh.insert(dh.end(), sec.upd.signs.begin(), sec.upd.signs.end());
+        if (sec.kind == Section::Update && sec.upd.left) {
+          sec.upd.sq_off = (int)lsq_n;
+          sec.upd.sq_n = ccqk::update_blocks(update_params(sec.upd));
+          lsq_n += sec.upd.sq_n;
+        }
         continue;
       }
+      if (sec.kind == Section::Copy) continue;
       for (auto& op : sec.ops) {
         const Tensor& C = tensors[op.out];
         const Tensor& A = tensors[op.a];
@@ -778,6 +857,7 @@ struct ccq_hip_ctx {
       }
     }
     ws.alloc(std::max<int64_t>(ws_need, 1));
+    if (lsq_n) lsq.alloc(lsq_n);
     HIPCHK(hipMalloc(&tables32, sizeof(int) * std::max<size_t>(host32.size(), 1)));
     if (!host32.empty())
       HIPCHK(hipMemcpy(tables32, host32.data(), sizeof(int) * host32.size(), hipMemcpyHostToDevice));
@@ -823,7 +903,7 @@ struct ccq_hip_ctx {
     }
   }
 
-  void run_update(const UpdateStep& u) {
+  ccqk::UpdateParams update_params(const UpdateStep& u) const {
     const Tensor& R = tensors[u.resid];
     ccqk::UpdateParams P{};
     P.rank = R.rank;
@@ -860,9 +940,29 @@ struct ccq_hip_ctx {
       P.vir_end[0] = vir_end[0];
       P.vir_end[1] = vir_end[1];
     }
-    double* t = tvec.p + blk_off.at(b);
+    return P;
+  }
+
+  void run_update(const UpdateStep& u) {
+    ccqk::UpdateParams P = update_params(u);
+    const Tensor& R = tensors[u.resid];
+    if (u.left) {
+      // update_l2 (src/update_lcc.f90:206-273): L -= LH / (Hbar_aa - Hbar_ii - E_cor + shift)
+      P.shift = shift - left_ecor;
+      double* l = lvec.p + blk_off.at(dimkey(u.block));
+      HIPCHK(ccqk::launch_update(stream, P, l, R.ptr, itab + u.dev_perm_off, dtab + u.dev_sign_off,
+                                 tens("cct3:HD.a").ptr, tens("cct3:HD.b").ptr, lsq.p + u.sq_off));
+      return;
+    }
+    double* t = tvec.p + blk_off.at(u.block);
     HIPCHK(ccqk::launch_update(stream, P, t, R.ptr, itab + u.dev_perm_off, dtab + u.dev_sign_off,
                                fdiag_a.p, fdiag_b.p));
+  }
+
+  const Tensor& tens(const std::string& id) const {
+    auto it = tindex.find(id);
+    if (it == tindex.end()) throw Fail("no tensor " + id);
+    return tensors[it->second];
   }
 
   void run_copy(const std::string& dst, const std::string& src) {
@@ -871,34 +971,53 @@ struct ccq_hip_ctx {
                              blk_size.at(src)));
   }
 
-  // A sweep is a fixed sequence of kernel launches on fixed buffers; it is
-  // captured once into a hipGraph and replayed (CCQ_HIP_GRAPH=0 disables).
+  // A sweep (and a left-CC step) is a fixed sequence of kernel launches on
+  // fixed buffers; it is captured once into a hipGraph and replayed
+  // (CCQ_HIP_GRAPH=0 disables).  key: the scalar baked into the launches.
   bool use_graph = true;
-  hipGraphExec_t graph_exec = nullptr;
-  hipGraph_t graph = nullptr;
+  struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    double key = 0.0;
+  };
+  std::map<std::string, Graph> graphs;
 
-  void run_sweep() {
+  static void drop_graph(Graph& G) {
+    if (G.x) (void)hipGraphExecDestroy(G.x);
+    if (G.g) (void)hipGraphDestroy(G.g);
+    G.x = nullptr;
+    G.g = nullptr;
+  }
+
+  void run_graphed(const std::string& prog, double key = 0.0) {
     if (!use_graph || profile) {
-      sweep();
+      run_program(prog);
       return;
     }
-    if (!graph_exec) {
+    Graph& G = graphs[prog];
+    if (G.x && G.key != key) drop_graph(G);
+    if (!G.x) {
       HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
       try {
-        sweep();
+        run_program(prog);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(stream, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
       }
-      HIPCHK(hipStreamEndCapture(stream, &graph));
-      HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+      HIPCHK(hipStreamEndCapture(stream, &G.g));
+      HIPCHK(hipGraphInstantiate(&G.x, G.g, nullptr, nullptr, 0));
+      G.key = key;
     }
-    HIPCHK(hipGraphLaunch(graph_exec, stream));
+    HIPCHK(hipGraphLaunch(G.x, stream));
   }
 
-  void sweep(int64_t nsec = -1) {
+  void run_sweep() { run_graphed("sweep"); }
+
+  void sweep(int64_t nsec = -1) { run_program("sweep", nsec); }
+
+  void run_program(const std::string& prog, int64_t nsec = -1) {
     if (!ints_ready) throw Fail("integrals not uploaded");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (profile) {
@@ -924,8 +1043,11 @@ struct ccq_hip_ctx {
     };
     int64_t isec = 0;
     for (auto& sec : sections) {
+      if (sec.prog != prog) continue;
       if (nsec >= 0 && isec++ >= nsec) break;
-      if (sec.kind == Section::Compute) {
+      if (sec.kind == Section::Cct3) {
+        timed("cct3:" + sec.upd.block, 0, 0, [&] { run_cct3(sec.upd); });
+      } else if (sec.kind == Section::Compute) {
         for (auto& op : sec.ops) timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
       } else if (sec.kind == Section::Update) {
         const Tensor& R = tensors[sec.upd.resid];
@@ -995,6 +1117,248 @@ struct ccq_hip_ctx {
     }
     return total;
   }
+
+  // -------------------------------------------------------------------------
+  // CC(t;3) tail
+
+  bool hbar_ready = false;
+
+  void add_inter(const std::string& id, std::vector<int64_t> d) {
+    Tensor t;
+    t.id = id;
+    t.kind = TKind::Inter;
+    t.rank = (int)d.size();
+    for (int i = 0; i < t.rank; ++i) t.dims[i] = d[i];
+    tindex[id] = (int)tensors.size();
+    tensors.push_back(t);
+  }
+
+  // H-bar one-body diagonals [occ | vir] per spin and cal_d3's arrays
+  // (src/mm_correct.f90:345-481), shapes [x][y][z] as there
+  void add_tail_tensors() {
+    add_inter("cct3:HD.a", {nc});
+    add_inter("cct3:HD.b", {nc});
+    add_inter("cct3:D3.a1", {va, oa, oa});
+    add_inter("cct3:D3.a2", {va, va, oa});
+    add_inter("cct3:D3.b1", {va, ob, oa});
+    add_inter("cct3:D3.b2", {vb, va, oa});
+    add_inter("cct3:D3.c1", {vb, ob, oa});
+    add_inter("cct3:D3.c2", {vb, va, ob});
+    add_inter("cct3:D3.d1", {vb, ob, ob});
+    add_inter("cct3:D3.d2", {vb, vb, ob});
+  }
+
+  // hbar2 + add_twobody_hbar (src/hbar.f90:10-720) from the current T1, T2
+  void hbar() {
+    if (!tail) throw Fail("H-bar needs the CC(t;3) program (level 6)");
+    run_program("hbar");
+    auto diag = [&](const char* src, int64_t n, double* dst) {
+      HIPCHK(ccqk::launch_diag(stream, tens(src).ptr, n, n, dst));
+    };
+    double* ha = tens("cct3:HD.a").ptr;
+    double* hb = tens("cct3:HD.b").ptr;
+    diag("cct3:HB1.oo.aa", oa, ha);
+    diag("cct3:HB1.vv.aa", va, ha + oa);
+    diag("cct3:HB1.oo.bb", ob, hb);
+    diag("cct3:HB1.vv.bb", vb, hb + ob);
+    HIPCHK(hipStreamSynchronize(stream));
+    hbar_ready = true;
+  }
+
+  void left_init(double ecor) {
+    if (!tail) throw Fail("left-CC needs the CC(t;3) program (level 6)");
+    left_ecor = ecor;
+    // ccq's l_vec starts from zero here
+    HIPCHK(hipMemsetAsync(lvec.p, 0, sizeof(double) * l_size, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  void left_update() {
+    if (!hbar_ready) throw Fail("H-bar not built (call ccq_hip_hbar first)");
+    run_graphed("left", left_ecor);
+  }
+
+  // calculate_left_energy (src/energy.f90:103-136): |LH| + E_cor
+  double left_energy() {
+    host_partial.resize((size_t)lsq_n);
+    HIPCHK(hipMemcpyAsync(host_partial.data(), lsq.p, sizeof(double) * lsq_n, hipMemcpyDeviceToHost,
+                          stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    double s = 0.0;
+    for (int64_t i = 0; i < lsq_n; ++i) s += host_partial[i];
+    return std::sqrt(s) + left_ecor;
+  }
+
+  // cal_d3 as gathered dot products: out[m] = scale sum_k X[xr[m] + xk[k]] Y[yr[m] + yk[k]]
+  // over the bare integrals (C order [p][q][r][s], occupied first per spin) and T2
+  void plan_d3() {
+    std::vector<int64_t> host;
+    const int64_t n1 = nc, n2 = nc * nc, n3 = nc * nc * nc;
+    auto put = [&](const std::vector<int64_t>& v) {
+      int64_t o = (int64_t)host.size();
+      host.insert(host.end(), v.begin(), v.end());
+      return o;
+    };
+    struct Spec {
+      const char* out;
+      const double* X;
+      const char* t2;
+      int64_t e0, e1, e2;     // output extents [x][y][z]
+      int64_t K;
+      std::function<int64_t(int64_t, int64_t, int64_t)> xr, yr;
+      std::function<int64_t(int64_t)> xk, yk;
+      double scale;
+    };
+    const double* Va = Vaa.p;
+    const double* Vm = Vab.p;
+    const double* Vb = vbb_ptr;
+    // d3d1's e loop starts at orbital occ_a + 2 (src/mm_correct.f90:459)
+    const int64_t e0 = std::min<int64_t>(vb, std::max<int64_t>(0, oa + 1 - ob));
+    std::vector<Spec> specs = {
+        // d3a1[a][j][i] = sum_e <ji||ea> t2aa[i][j][a][e]
+        {"cct3:D3.a1", Va, "t2aa", va, oa, oa, va,
+         [&](int64_t a, int64_t j, int64_t i) { return j * n3 + i * n2 + oa * n1 + oa + a; },
+         [&](int64_t a, int64_t j, int64_t i) { return ((i * oa + j) * va + a) * va; },
+         [&](int64_t e) { return e * n1; }, [&](int64_t e) { return e; }, 1.0},
+        // d3a2[b][a][i] = -sum_m <mi||ba> t2aa[i][m][a][b]
+        {"cct3:D3.a2", Va, "t2aa", va, va, oa, oa,
+         [&](int64_t b, int64_t a, int64_t i) { return i * n2 + (oa + b) * n1 + oa + a; },
+         [&](int64_t b, int64_t a, int64_t i) { return i * oa * va * va + a * va + b; },
+         [&](int64_t m) { return m * n3; }, [&](int64_t m) { return m * va * va; }, -1.0},
+        // d3b1[a][j][i] = sum_e <i j|a e>_ab t2ab[i][j][a][e]
+        {"cct3:D3.b1", Vm, "t2ab", va, ob, oa, vb,
+         [&](int64_t a, int64_t j, int64_t i) { return i * n3 + j * n2 + (oa + a) * n1 + ob; },
+         [&](int64_t a, int64_t j, int64_t i) { return ((i * ob + j) * va + a) * vb; },
+         [&](int64_t e) { return e; }, [&](int64_t e) { return e; }, 1.0},
+        // d3b2[b][a][i] = -sum_m <i m|a b>_ab t2ab[i][m][a][b]
+        {"cct3:D3.b2", Vm, "t2ab", vb, va, oa, ob,
+         [&](int64_t b, int64_t a, int64_t i) { return i * n3 + (oa + a) * n1 + ob + b; },
+         [&](int64_t b, int64_t a, int64_t i) { return i * ob * va * vb + a * vb + b; },
+         [&](int64_t m) { return m * n2; }, [&](int64_t m) { return m * va * vb; }, -1.0},
+        // d3c1[b][j][i] = sum_e <i j|e b>_ab t2ab[i][j][e][b]
+        {"cct3:D3.c1", Vm, "t2ab", vb, ob, oa, va,
+         [&](int64_t b, int64_t j, int64_t i) { return i * n3 + j * n2 + oa * n1 + ob + b; },
+         [&](int64_t b, int64_t j, int64_t i) { return (i * ob + j) * va * vb + b; },
+         [&](int64_t e) { return e * n1; }, [&](int64_t e) { return e * vb; }, 1.0},
+        // d3c2[b][a][j] = -sum_m <m j|a b>_ab t2ab[m][j][a][b]
+        {"cct3:D3.c2", Vm, "t2ab", vb, va, ob, oa,
+         [&](int64_t b, int64_t a, int64_t j) { return j * n2 + (oa + a) * n1 + ob + b; },
+         [&](int64_t b, int64_t a, int64_t j) { return j * va * vb + a * vb + b; },
+         [&](int64_t m) { return m * n3; }, [&](int64_t m) { return m * ob * va * vb; }, -1.0},
+        // d3d1[a][j][i] = sum_{e >= e0} <ji||ea> t2bb[i][j][a][e]
+        {"cct3:D3.d1", Vb, "t2bb", vb, ob, ob, vb - e0,
+         [&](int64_t a, int64_t j, int64_t i) { return j * n3 + i * n2 + (ob + e0) * n1 + ob + a; },
+         [&](int64_t a, int64_t j, int64_t i) { return ((i * ob + j) * vb + a) * vb + e0; },
+         [&](int64_t e) { return e * n1; }, [&](int64_t e) { return e; }, 1.0},
+        // d3d2[b][a][i] = -sum_m <mi||ba> t2bb[i][m][a][b]
+        {"cct3:D3.d2", Vb, "t2bb", vb, vb, ob, ob,
+         [&](int64_t b, int64_t a, int64_t i) { return i * n2 + (ob + b) * n1 + ob + a; },
+         [&](int64_t b, int64_t a, int64_t i) { return i * ob * vb * vb + a * vb + b; },
+         [&](int64_t m) { return m * n3; }, [&](int64_t m) { return m * vb * vb; }, -1.0},
+    };
+    for (auto& sp : specs) {
+      Had h{};
+      h.out = tindex.at(sp.out);
+      h.X = sp.X;
+      h.Y = tvec.p + blk_off.at(sp.t2);
+      h.M = sp.e0 * sp.e1 * sp.e2;
+      h.K = sp.K;
+      h.scale = sp.scale;
+      std::vector<int64_t> xr, yr, xk, yk;
+      for (int64_t x = 0; x < sp.e0; ++x)
+        for (int64_t y = 0; y < sp.e1; ++y)
+          for (int64_t z = 0; z < sp.e2; ++z) {
+            xr.push_back(sp.xr(x, y, z));
+            yr.push_back(sp.yr(x, y, z));
+          }
+      for (int64_t k = 0; k < sp.K; ++k) {
+        xk.push_back(sp.xk(k));
+        yk.push_back(sp.yk(k));
+      }
+      // every gathered element must lie inside its array
+      const int64_t xlim = nc * nc * nc * nc, ylim = blk_size.at(sp.t2);
+      int64_t xmax = 0, ymax = 0;
+      for (auto v : xr) xmax = std::max(xmax, v);
+      for (auto v : yr) ymax = std::max(ymax, v);
+      int64_t xkm = 0, ykm = 0;
+      for (auto v : xk) xkm = std::max(xkm, v);
+      for (auto v : yk) ykm = std::max(ykm, v);
+      if (h.M > 0 && h.K > 0 && (xmax + xkm >= xlim || ymax + ykm >= ylim))
+        throw Fail(std::string("cal_d3 table out of range for ") + sp.out);
+      h.xr = put(xr);
+      h.yr = put(yr);
+      h.xk = put(xk);
+      h.yk = put(yk);
+      hads.push_back(h);
+    }
+    HIPCHK(hipMalloc(&d3tab, sizeof(int64_t) * std::max<size_t>(host.size(), 1)));
+    if (!host.empty())
+      HIPCHK(hipMemcpy(d3tab, host.data(), sizeof(int64_t) * host.size(), hipMemcpyHostToDevice));
+  }
+
+  void run_cct3(const UpdateStep& u) {
+    const Tensor& L = tensors[u.resid];
+    const Tensor& M = tensors[u.m3];
+    ccqk::Cct3Params P{};
+    const std::string sp = u.block.substr(2);
+    int cs = 0;
+    for (char c : sp) cs += c == 'b';
+    P.cs = cs;
+    P.nimg = u.nimg;
+    P.total = 1;
+    for (int d = 0; d < 6; ++d) {
+      P.dims[d] = L.dims[d];
+      P.strides[d] = L.strides[d];
+      P.total *= L.dims[d];
+      if (M.dims[d] != L.dims[d] || M.strides[d] != L.strides[d]) throw Fail("L3 / M3 layouts differ");
+    }
+    P.oa = oa;
+    P.ob = ob;
+    P.va = va;
+    P.vb = vb;
+    P.occ_start = occ_start;
+    P.vir_end[0] = vir_end[0];
+    P.vir_end[1] = vir_end[1];
+    P.fd[0] = fdiag_a.p;
+    P.fd[1] = fdiag_b.p;
+    P.hd[0] = tens("cct3:HD.a").ptr;
+    P.hd[1] = tens("cct3:HD.b").ptr;
+    P.ovov[0][0] = tens("cct3:HB.ovov.aaaa").ptr;
+    P.ovov[1][0] = tens("cct3:HB.ovov.baba").ptr;
+    P.ovov[0][1] = tens("cct3:HB.ovov.abab").ptr;
+    P.ovov[1][1] = tens("cct3:HB.ovov.bbbb").ptr;
+    const char* sfx[3] = {"aaaa", "abab", "bbbb"};
+    for (int x = 0; x < 3; ++x) {
+      P.oooo[x] = tens(std::string("cct3:HB.oooo.") + sfx[x]).ptr;
+      P.vvvv[x] = tens(std::string("cct3:HB.vvvv.") + sfx[x]).ptr;
+    }
+    const char* d3n[8] = {"a1", "a2", "b1", "b2", "c1", "c2", "d1", "d2"};
+    for (int x = 0; x < 8; ++x) P.d3[x] = tens(std::string("cct3:D3.") + d3n[x]).ptr;
+    HIPCHK(ccqk::launch_cct3(stream, P, L.ptr, M.ptr, itab + u.dev_perm_off, dtab + u.dev_sign_off,
+                             cpart.p + (int64_t)cs * kPartialBlocks * 4, kPartialBlocks));
+  }
+
+  // crcc23 (src/mm_correct.f90:11-343): e23[A..D]
+  void cct3(double* e23) {
+    if (!hbar_ready) throw Fail("H-bar not built (call ccq_hip_hbar first)");
+    if (hads.empty()) plan_d3();
+    for (auto& h : hads)
+      HIPCHK(ccqk::launch_hadamard_sum(stream, h.M, h.K, h.X, d3tab + h.xr, d3tab + h.xk, h.Y,
+                                       d3tab + h.yr, d3tab + h.yk, tensors[h.out].ptr, h.scale));
+    HIPCHK(hipMemsetAsync(cpart.p, 0, sizeof(double) * cpart.n, stream));
+    run_program("cct3");
+    host_partial.resize((size_t)cpart.n);
+    HIPCHK(hipMemcpyAsync(host_partial.data(), cpart.p, sizeof(double) * cpart.n,
+                          hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int x = 0; x < 4; ++x) e23[x] = 0.0;
+    for (int cs = 0; cs < 4; ++cs)
+      for (int b = 0; b < kPartialBlocks; ++b)
+        for (int x = 0; x < 4; ++x) e23[x] += host_partial[((size_t)cs * kPartialBlocks + b) * 4 + x];
+  }
+
+  // the DIIS vector the ring works on: t_vec, or l_vec in left mode
+  int64_t diis_len() const { return left_mode ? l_size : vec_size; }
 
   int64_t slot_of(int64_t iter) const {
     int64_t s = iter % (diis + 1);
@@ -1071,9 +1435,11 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (ctx->oa < 0 || ctx->ob < 0 || ctx->va < 0 || ctx->vb < 0 || ctx->nc <= 0)
       throw Fail("inconsistent orbital counts");
     ctx->level = (int)opts->level;
-    if (ctx->level < 2 || ctx->level > 5)
-      throw Fail("level must be 2 (CCSD), 3 (CCSDT), 4 (CCSDTQ) or 5 (CCSDt)");
-    if (ctx->level == 5) {
+    if (ctx->level < 2 || ctx->level > 6)
+      throw Fail("level must be 2 (CCSD), 3 (CCSDT), 4 (CCSDTQ), 5 (CCSDt) or 6 (CC(t;3))");
+    if (ctx->level >= 5) {
+      // CC(t;3) = CCSDt, then H-bar, left-CCSD and crcc23 (src/calc_driver.f90:60-72)
+      ctx->tail = ctx->level == 6;
       // CCSDt: CCSDT equations with active-space triples, no closed-shell
       // shortcuts (update_clusters_t3_opt, src/update_cc.F90:325-2191)
       if (opts->act_occ == -1 && opts->act_unocc == -1)
@@ -1110,9 +1476,22 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     HIPCHK(hipMemsetAsync(ctx->tvec.p, 0, sizeof(double) * ctx->dev_t, ctx->stream));
     ctx->ring.alloc((int64_t)(ctx->diis + 1) * ctx->vec_size);
     if (ctx->active) ctx->staging.alloc(ctx->ext_size);
+    if (ctx->tail) {
+      ctx->l_size = ctx->blk_off["t3aaa"];
+      ctx->lvec.alloc(ctx->l_size);
+      HIPCHK(hipMemsetAsync(ctx->lvec.p, 0, sizeof(double) * ctx->l_size, ctx->stream));
+      ctx->cpart.alloc(4 * kPartialBlocks * 4);
+      if (ctx->l_size > ctx->vec_size) throw Fail("DIIS ring too small for l_vec");
+    }
     ctx->partial.alloc(kPartialBlocks * 128);
     if (ctx->level == 2) ctx->load_ir(ctx->rhf ? kIR_CCSD_RHF : kIR_CCSD_UHF);
-    else if (ctx->active) ctx->load_ir(kIR_CCSDT_ACT_UHF);
+    else if (ctx->active) {
+      ctx->load_ir(kIR_CCSDT_ACT_UHF);
+      if (ctx->tail) {
+        ctx->load_ir(kIR_CCT3_UHF, "cct3:");
+        ctx->add_tail_tensors();
+      }
+    }
     else if (ctx->level == 3) ctx->load_ir(ctx->rhf ? kIR_CCSDT_RHF : kIR_CCSDT_UHF);
     else {
 #ifdef CCQ_HAVE_CCSDTQ
@@ -1252,6 +1631,11 @@ int ccq_hip_energy(ccq_hip_ctx* ctx, double* e) {
 int ccq_hip_diis_record(ccq_hip_ctx* ctx, int64_t iter) {
   return guarded(ctx, [&] {
     const int64_t s = ctx->slot_of(iter);
+    if (ctx->left_mode) {
+      HIPCHK(hipMemcpyAsync(ctx->ring.p + s * ctx->l_size, ctx->lvec.p, sizeof(double) * ctx->l_size,
+                            hipMemcpyDeviceToDevice, ctx->stream));
+      return;
+    }
     if (ctx->active) {
       // the DIIS vector is ccq's packed t_vec (conv%vec_size = t_size)
       ctx->pack(ctx->ring.p + s * ctx->vec_size);
@@ -1264,10 +1648,10 @@ int ccq_hip_diis_record(ccq_hip_ctx* ctx, int64_t iter) {
 
 int ccq_hip_residuum(ccq_hip_ctx* ctx, int64_t iter, double* res) {
   return guarded(ctx, [&] {
-    const double* cur = ctx->ring.p + ctx->slot_of(iter) * ctx->vec_size;
-    const double* prev = iter == 1 ? nullptr : ctx->ring.p + ctx->slot_of(iter - 1) * ctx->vec_size;
-    HIPCHK(ccqk::launch_diff_sq(ctx->stream, prev, cur, ctx->vec_size, ctx->partial.p,
-                                kPartialBlocks));
+    const int64_t len = ctx->diis_len();
+    const double* cur = ctx->ring.p + ctx->slot_of(iter) * len;
+    const double* prev = iter == 1 ? nullptr : ctx->ring.p + ctx->slot_of(iter - 1) * len;
+    HIPCHK(ccqk::launch_diff_sq(ctx->stream, prev, cur, len, ctx->partial.p, kPartialBlocks));
     *res = std::sqrt(std::sqrt(ctx->reduce_partials(kPartialBlocks)));
   });
 }
@@ -1276,7 +1660,8 @@ int ccq_hip_diis_extrapolate(ccq_hip_ctx* ctx) {
   return guarded(ctx, [&] {
     const int n = ctx->diis;
     const int np = n * (n + 1) / 2;
-    HIPCHK(ccqk::launch_diis_gram(ctx->stream, ctx->ring.p, ctx->vec_size, n, ctx->partial.p,
+    const int64_t len = ctx->diis_len();
+    HIPCHK(ccqk::launch_diis_gram(ctx->stream, ctx->ring.p, len, n, ctx->partial.p,
                                   kPartialBlocks));
     ctx->host_partial.resize((size_t)kPartialBlocks * np);
     HIPCHK(hipMemcpyAsync(ctx->host_partial.data(), ctx->partial.p,
@@ -1302,13 +1687,62 @@ int ccq_hip_diis_extrapolate(ccq_hip_ctx* ctx) {
     if (!solve_dense(B, c, m)) throw Fail("calc_diis: DIIS error in DGESV.");
     ccqk::Coeffs cf{};
     for (int k = 0; k < n; ++k) cf.c[k] = c[k];
-    if (ctx->active) {
+    if (ctx->left_mode) {
+      HIPCHK(ccqk::launch_lincomb(ctx->stream, ctx->lvec.p, ctx->ring.p, len, n, cf));
+    } else if (ctx->active) {
       HIPCHK(ccqk::launch_lincomb(ctx->stream, ctx->staging.p, ctx->ring.p, ctx->vec_size, n, cf));
       ctx->unpack(ctx->staging.p);
     } else {
       HIPCHK(ccqk::launch_lincomb(ctx->stream, ctx->tvec.p, ctx->ring.p, ctx->vec_size, n, cf));
     }
   });
+}
+
+int ccq_hip_hbar(ccq_hip_ctx* ctx) {
+  return guarded(ctx, [&] { ctx->hbar(); });
+}
+
+int ccq_hip_left_init(ccq_hip_ctx* ctx, double e_cor) {
+  return guarded(ctx, [&] { ctx->left_init(e_cor); });
+}
+
+int ccq_hip_left_update(ccq_hip_ctx* ctx) {
+  return guarded(ctx, [&] { ctx->left_update(); });
+}
+
+int ccq_hip_left_energy(ccq_hip_ctx* ctx, double* e) {
+  return guarded(ctx, [&] { *e = ctx->left_energy(); });
+}
+
+int ccq_hip_left_mode(ccq_hip_ctx* ctx, int64_t on) {
+  return guarded(ctx, [&] {
+    if (on && !ctx->tail) throw Fail("left mode needs the CC(t;3) program (level 6)");
+    ctx->left_mode = on != 0;
+  });
+}
+
+int64_t ccq_hip_l_size(const ccq_hip_ctx* ctx) { return ctx ? ctx->l_size : -1; }
+
+int ccq_hip_set_l(ccq_hip_ctx* ctx, const double* l_vec, int64_t l_size) {
+  return guarded(ctx, [&] {
+    if (!ctx->tail || l_size != ctx->l_size) throw Fail("l_size mismatch");
+    HIPCHK(hipMemcpyAsync(ctx->lvec.p, l_vec, sizeof(double) * l_size, hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ccq_hip_get_l(ccq_hip_ctx* ctx, double* l_vec, int64_t l_size) {
+  return guarded(ctx, [&] {
+    if (!ctx->tail || l_size != ctx->l_size) throw Fail("l_size mismatch");
+    HIPCHK(hipMemcpyAsync(l_vec, ctx->lvec.p, sizeof(double) * l_size, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ccq_hip_cct3(ccq_hip_ctx* ctx, double* e23) {
+  return guarded(ctx, [&] { ctx->cct3(e23); });
 }
 
 int ccq_hip_profile(ccq_hip_ctx* ctx, int64_t enable) {

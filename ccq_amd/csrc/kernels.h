@@ -59,6 +59,33 @@ struct UpdateParams {
   int64_t vir_end[2] = {0, 0};
 };
 
+// CC(t;3) triples correction of one spin case (crcc23, src/mm_correct.f90:11-343).
+// The block has extents dims[0..5] (occupied, occupied, occupied, virtual x3)
+// with spin pattern cs: 0 aaa, 1 aab, 2 abb, 3 bbb.  L3 / M3 are residual
+// accumulators with images folded (the value at a tuple is the image average,
+// as in jacobi_update).  Only canonical tuples outside the P space count.
+// Diagonals, 0-based correlated indices [occ | vir] per spin:
+//   fd[s]: Fock; hd[s]: H-bar one-body.
+// Two-body H-bar blocks (dense C order) whose diagonals enter D_C / D_D:
+//   ovov[so][sv]: spins (so, sv, so, sv); oooo / vvvv [0 aaaa, 1 abab, 2 bbbb].
+// d3[0..7]: cal_d3's d3a1, d3a2, d3b1, d3b2, d3c1, d3c2, d3d1, d3d2.
+struct Cct3Params {
+  int cs;
+  int nimg;
+  int64_t total;
+  int64_t dims[6];
+  int64_t strides[6];
+  int64_t oa, ob, va, vb;
+  int64_t occ_start;
+  int64_t vir_end[2];
+  const double* fd[2];
+  const double* hd[2];
+  const double* ovov[2][2];
+  const double* oooo[3];
+  const double* vvvv[3];
+  const double* d3[8];
+};
+
 struct EnergyParams {
   const double* V;       // full-range base array
   int64_t vs0, vs1, vs2, vs3;   // strides of V
@@ -95,9 +122,22 @@ hipError_t launch_axpy_flat(hipStream_t s, int64_t n, const double* A, double* C
                             int accumulate);
 hipError_t launch_fill(hipStream_t s, double* x, int64_t n, double v);
 hipError_t launch_copy(hipStream_t s, double* y, const double* x, int64_t n);
+// sumsq (optional): per-block partial sums of |G| R^2 over the canonical
+// tuples updated (the squared norm of the full-storage residual); one entry
+// per block of the launch, update_blocks(P) of them.
 hipError_t launch_update(hipStream_t s, const UpdateParams& P, double* t, const double* U,
                          const int* perms, const double* signs, const double* fa,
-                         const double* fb);
+                         const double* fb, double* sumsq = nullptr);
+int update_blocks(const UpdateParams& P);
+// out[m] = scale * sum_{k < K} X[xr[m] + xk[k]] * Y[yr[m] + yk[k]]
+hipError_t launch_hadamard_sum(hipStream_t s, int64_t M, int64_t K, const double* X,
+                               const int64_t* xr, const int64_t* xk, const double* Y,
+                               const int64_t* yr, const int64_t* yk, double* out, double scale);
+// partial[block * 4 + x]: per-block sums of l3 * m3 / D_x, x = A, B, C, D
+hipError_t launch_cct3(hipStream_t s, const Cct3Params& P, const double* L3, const double* M3,
+                       const int* perms, const double* signs, double* partial, int blocks);
+// dst[i] = src[i * (ld + 1)], i < n (diagonal of a square block with row stride ld)
+hipError_t launch_diag(hipStream_t s, const double* src, int64_t n, int64_t ld, double* dst);
 hipError_t launch_energy_pairs(hipStream_t s, const EnergyParams& P, double* partial, int blocks);
 hipError_t launch_dot_strided(hipStream_t s, int64_t rows, int64_t cols, const double* a,
                               int64_t lda, const double* b, double* partial, int blocks);

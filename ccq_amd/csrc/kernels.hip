@@ -469,10 +469,16 @@ __global__ void fill(double* __restrict__ x, int64_t n, double v) {
 
 // One thread per element of the full block; only canonical tuples (strictly
 // increasing inside every antisymmetric group) do work.
-__global__ void jacobi_update(UpdateParams P, double* __restrict__ t, const double* __restrict__ U,
-                              const int* __restrict__ perms, const double* __restrict__ signs,
-                              const double* __restrict__ fdiag_a, const double* __restrict__ fdiag_b) {
+template <bool SUMSQ>
+__global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __restrict__ t,
+                                                     const double* __restrict__ U,
+                                                     const int* __restrict__ perms,
+                                                     const double* __restrict__ signs,
+                                                     const double* __restrict__ fdiag_a,
+                                                     const double* __restrict__ fdiag_b,
+                                                     double* __restrict__ sumsq) {
   const int rank = P.rank;
+  double sq = 0.0;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < P.total;
        e += (int64_t)gridDim.x * blockDim.x) {
     int64_t idx[8];
@@ -505,6 +511,7 @@ __global__ void jacobi_update(UpdateParams P, double* __restrict__ t, const doub
       R += signs[g] * U[off];
     }
     R /= (double)P.nimg;
+    if (SUMSQ) sq += (double)P.nimg * R * R;
     double D = 0.0;
     for (int d = 0; d < rank; ++d)
       if (P.isvir[d]) D += (P.spin[d] ? fdiag_b : fdiag_a)[P.occ[d] + idx[d]];
@@ -520,6 +527,188 @@ __global__ void jacobi_update(UpdateParams P, double* __restrict__ t, const doub
       t[off] = signs[g] * tn;
     }
   }
+  if (SUMSQ) {
+    __shared__ double red[256];
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) sumsq[blockIdx.x] = red[0];
+  }
+}
+
+// out[m] = scale * sum_k X[xr[m] + xk[k]] * Y[yr[m] + yk[k]]
+__global__ __launch_bounds__(256) void hadamard_sum(int64_t M, int64_t K, const double* __restrict__ X,
+                                                    const int64_t* __restrict__ xr,
+                                                    const int64_t* __restrict__ xk,
+                                                    const double* __restrict__ Y,
+                                                    const int64_t* __restrict__ yr,
+                                                    const int64_t* __restrict__ yk,
+                                                    double* __restrict__ out, double scale) {
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < M;
+       m += (int64_t)gridDim.x * blockDim.x) {
+    const double* x = X + xr[m];
+    const double* y = Y + yr[m];
+    double acc = 0.0;
+    for (int64_t k = 0; k < K; ++k) acc += x[xk[k]] * y[yk[k]];
+    out[m] = scale * acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void diag_copy(const double* __restrict__ src, int64_t n, int64_t ld,
+                                                 double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i * (ld + 1)];
+}
+
+// H-bar two-body diagonals of crcc23 (src/mm_correct.f90:116-330)
+struct Cct3Diag {
+  const Cct3Params& P;
+  __device__ int64_t no(int s) const { return s ? P.ob : P.oa; }
+  __device__ int64_t nv(int s) const { return s ? P.vb : P.va; }
+  // h2(a,i,a,i) = -<ia||ai> = <ia||ia>, block ovov (so, sv, so, sv)
+  __device__ double vo(int sv, int64_t a, int so, int64_t i) const {
+    const int64_t o = no(so), v = nv(sv);
+    return P.ovov[so][sv][((i * v + a) * o + i) * v + a];
+  }
+  // h2(p,q,p,q) of a same-kind pair; mixed spins live in the abab block with
+  // the alpha index first
+  __device__ double pair(const double* const* blk, int64_t n_a, int64_t n_b, int s1, int64_t p,
+                         int s2, int64_t q) const {
+    if (s1 == s2) {
+      const int64_t n = s1 ? n_b : n_a;
+      return blk[s1 ? 2 : 0][((p * n + q) * n + p) * n + q];
+    }
+    const int64_t x = s1 ? q : p, y = s1 ? p : q;   // x alpha, y beta
+    return blk[1][((x * n_b + y) * n_a + x) * n_b + y];
+  }
+  __device__ double oo(int s1, int64_t j, int s2, int64_t i) const {
+    return pair(P.oooo, P.oa, P.ob, s1, j, s2, i);
+  }
+  __device__ double vv(int s1, int64_t b, int s2, int64_t a) const {
+    return pair(P.vvvv, P.va, P.vb, s1, b, s2, a);
+  }
+};
+
+// d3 arrays are [x][y][z] with the extents of cal_d3 (src/mm_correct.f90:345-481)
+__device__ __forceinline__ double d3at(const double* d, int64_t n1, int64_t n2, int64_t x, int64_t y,
+                                       int64_t z) {
+  return d[(x * n1 + y) * n2 + z];
+}
+
+__global__ __launch_bounds__(256) void cct3_correction(Cct3Params P, const double* __restrict__ L3,
+                                                       const double* __restrict__ M3,
+                                                       const int* __restrict__ perms,
+                                                       const double* __restrict__ signs,
+                                                       double* __restrict__ partial) {
+  __shared__ double red[4][256];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int cs = P.cs;
+  // spins of (i, j, k) and of (a, b, c)
+  int sp[3];
+  for (int x = 0; x < 3; ++x) sp[x] = x >= 3 - cs ? 1 : 0;
+  const Cct3Diag H{P};
+  const int64_t oa = P.oa, ob = P.ob, va = P.va, vb = P.vb;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < P.total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t idx[6];
+    int64_t r = e;
+    for (int d = 5; d >= 0; --d) {
+      idx[d] = r % P.dims[d];
+      r /= P.dims[d];
+    }
+    // canonical: strictly increasing inside same-spin runs
+    bool canon = true;
+    for (int x = 0; x < 2; ++x)
+      if (sp[x] == sp[x + 1] && (idx[x] >= idx[x + 1] || idx[3 + x] >= idx[4 + x])) canon = false;
+    if (!canon) continue;
+    // outside the P space: no active occupied or no active virtual
+    bool occ_act = false, vir_act = false;
+    for (int x = 0; x < 3; ++x) {
+      occ_act |= idx[x] >= P.occ_start;
+      vir_act |= idx[3 + x] < P.vir_end[sp[x]];
+    }
+    if (occ_act && vir_act) continue;
+    double l = 0.0, m = 0.0;
+    for (int g = 0; g < P.nimg; ++g) {
+      int64_t off = 0;
+      for (int d = 0; d < 6; ++d) off += idx[d] * P.strides[perms[g * 8 + d]];
+      l += signs[g] * L3[off];
+      m += signs[g] * M3[off];
+    }
+    const double inv = 1.0 / (double)P.nimg;
+    const double lm = (l * inv) * (m * inv);
+    double da = 0.0, db = 0.0;
+    for (int x = 0; x < 3; ++x) {
+      const int s = sp[x];
+      const double* f = P.fd[s];
+      const double* h = P.hd[s];
+      const int64_t o = s ? ob : oa;
+      da += f[idx[x]] - f[o + idx[3 + x]];
+      db += h[idx[x]] - h[o + idx[3 + x]];
+    }
+    double dc = db;
+    for (int x = 0; x < 3; ++x)
+      for (int y = 0; y < 3; ++y) dc += H.vo(sp[x], idx[3 + x], sp[y], idx[y]);
+    for (int x = 0; x < 3; ++x)
+      for (int y = x + 1; y < 3; ++y) {
+        dc -= H.oo(sp[y], idx[y], sp[x], idx[x]);
+        dc -= H.vv(sp[y], idx[3 + y], sp[x], idx[3 + x]);
+      }
+    const int64_t i = idx[0], j = idx[1], k = idx[2], a = idx[3], b = idx[4], c = idx[5];
+    const double* const* D = P.d3;
+    double x3 = 0.0;
+    if (cs == 0) {
+      // d3a1 (va, oa, oa), d3a2 (va, va, oa)
+      auto A1 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[0], oa, oa, p, q, s); };
+      auto A2 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[1], va, oa, p, q, s); };
+      x3 = A1(a, j, i) + A1(a, k, i) + A1(a, k, j) + A1(b, j, i) + A1(b, k, i) + A1(b, k, j) +
+           A1(c, j, i) + A1(c, k, i) + A1(c, k, j) - A2(b, a, i) - A2(b, a, j) - A2(b, a, k) -
+           A2(c, a, i) - A2(c, a, j) - A2(c, a, k) - A2(c, b, i) - A2(c, b, j) - A2(c, b, k);
+    } else if (cs == 1) {
+      auto A1 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[0], oa, oa, p, q, s); };
+      auto A2 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[1], va, oa, p, q, s); };
+      auto B1 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[2], ob, oa, p, q, s); };
+      auto B2 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[3], va, oa, p, q, s); };
+      auto C1 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[4], ob, oa, p, q, s); };
+      auto C2 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[5], va, ob, p, q, s); };
+      x3 = A1(a, j, i) + B1(a, k, i) + B1(a, k, j) + A1(b, j, i) + B1(b, k, i) + B1(b, k, j) +
+           C1(c, k, i) + C1(c, k, j) - A2(b, a, i) - A2(b, a, j) - B2(c, a, i) - B2(c, a, j) -
+           C2(c, a, k) - B2(c, b, i) - B2(c, b, j) - C2(c, b, k);
+    } else if (cs == 2) {
+      auto B1 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[2], ob, oa, p, q, s); };
+      auto B2 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[3], va, oa, p, q, s); };
+      auto C1 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[4], ob, oa, p, q, s); };
+      auto C2 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[5], va, ob, p, q, s); };
+      auto D1 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[6], ob, ob, p, q, s); };
+      auto D2 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[7], vb, ob, p, q, s); };
+      x3 = B1(a, j, i) + B1(a, k, i) + C1(b, j, i) + C1(b, k, i) + D1(b, k, j) + C1(c, j, i) +
+           C1(c, k, i) + D1(c, k, j) - B2(b, a, i) - C2(b, a, j) - C2(b, a, k) - B2(c, a, i) -
+           C2(c, a, j) - C2(c, a, k) - D2(c, b, j) - D2(c, b, k);
+    } else {
+      auto D1 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[6], ob, ob, p, q, s); };
+      auto D2 = [&](int64_t p, int64_t q, int64_t s) { return d3at(D[7], vb, ob, p, q, s); };
+      x3 = D1(a, j, i) + D1(a, k, i) + D1(a, k, j) + D1(b, j, i) + D1(b, k, i) + D1(b, k, j) +
+           D1(c, j, i) + D1(c, k, i) + D1(c, k, j) - D2(b, a, i) - D2(b, a, j) - D2(b, a, k) -
+           D2(c, a, i) - D2(c, a, j) - D2(c, a, k) - D2(c, b, i) - D2(c, b, j) - D2(c, b, k);
+    }
+    const double dd = dc + x3;
+    acc[0] += lm / da;
+    acc[1] += lm / db;
+    acc[2] += lm / dc;
+    acc[3] += lm / dd;
+  }
+  for (int x = 0; x < 4; ++x) red[x][threadIdx.x] = acc[x];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int x = 0; x < 4; ++x) red[x][threadIdx.x] += red[x][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int x = 0; x < 4; ++x) partial[blockIdx.x * 4 + x] = red[x][0];
 }
 
 // Energy pieces for one spin case over (m, n, e, f):
@@ -901,11 +1090,39 @@ hipError_t launch_fill(hipStream_t s, double* x, int64_t n, double v) {
   return hipGetLastError();
 }
 
+int update_blocks(const UpdateParams& P) { return grid_for(P.total, 256); }
+
 hipError_t launch_update(hipStream_t s, const UpdateParams& P, double* t, const double* U,
                          const int* perms, const double* signs, const double* fa,
-                         const double* fb) {
-  hipLaunchKernelGGL(jacobi_update, dim3(grid_for(P.total, 256)), dim3(256), 0, s, P, t, U, perms,
-                     signs, fa, fb);
+                         const double* fb, double* sumsq) {
+  if (sumsq)
+    hipLaunchKernelGGL(jacobi_update<true>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U, perms,
+                       signs, fa, fb, sumsq);
+  else
+    hipLaunchKernelGGL(jacobi_update<false>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U,
+                       perms, signs, fa, fb, sumsq);
+  return hipGetLastError();
+}
+
+hipError_t launch_hadamard_sum(hipStream_t s, int64_t M, int64_t K, const double* X,
+                               const int64_t* xr, const int64_t* xk, const double* Y,
+                               const int64_t* yr, const int64_t* yk, double* out, double scale) {
+  if (M <= 0) return hipSuccess;
+  hipLaunchKernelGGL(hadamard_sum, dim3(grid_for(M, 256)), dim3(256), 0, s, M, K, X, xr, xk, Y, yr,
+                     yk, out, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_cct3(hipStream_t s, const Cct3Params& P, const double* L3, const double* M3,
+                       const int* perms, const double* signs, double* partial, int blocks) {
+  hipLaunchKernelGGL(cct3_correction, dim3(blocks), dim3(256), 0, s, P, L3, M3, perms, signs,
+                     partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag(hipStream_t s, const double* src, int64_t n, int64_t ld, double* dst) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(diag_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, n, ld, dst);
   return hipGetLastError();
 }
 

@@ -20,7 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libccq_hip.so")
 
-LEVELS = {"CCSD": 2, "CCSDT": 3, "CCSDTQ": 4, "CCSDt": 5}
+LEVELS = {"CCSD": 2, "CCSDT": 3, "CCSDTQ": 4, "CCSDt": 5, "CC(t;3)": 6}
 
 
 class EngineError(RuntimeError):
@@ -54,6 +54,15 @@ EXPORTS = {
     "ccq_hip_diis_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_residuum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]),
     "ccq_hip_diis_extrapolate": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccq_hip_hbar": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccq_hip_left_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "ccq_hip_left_update": (ctypes.c_int, [ctypes.c_void_p]),
+    "ccq_hip_left_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_left_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_l_size": (ctypes.c_int64, [ctypes.c_void_p]),
+    "ccq_hip_set_l": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_get_l": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_cct3": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "ccq_hip_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_profile_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "ccq_hip_debug_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -101,6 +110,7 @@ class Engine:
         self.calc_type = calc_type
         self.t_size = int(self.lib.ccq_hip_t_size(self.h))
         self.t4_size = int(self.lib.ccq_hip_t4_size(self.h))
+        self.l_size = int(self.lib.ccq_hip_l_size(self.h))
 
     def _check(self, rc: int, what: str):
         if rc != 0:
@@ -150,6 +160,40 @@ class Engine:
 
     def diis_extrapolate(self):
         self._check(self.lib.ccq_hip_diis_extrapolate(self.h), "diis_extrapolate")
+
+    # CC(t;3) tail (calc_type "CC(t;3)")
+
+    def hbar(self):
+        self._check(self.lib.ccq_hip_hbar(self.h), "hbar")
+
+    def left_init(self, e_cor: float):
+        self._check(self.lib.ccq_hip_left_init(self.h, float(e_cor)), "left_init")
+
+    def left_update(self):
+        self._check(self.lib.ccq_hip_left_update(self.h), "left_update")
+
+    def left_energy(self) -> float:
+        e = ctypes.c_double()
+        self._check(self.lib.ccq_hip_left_energy(self.h, ctypes.byref(e)), "left_energy")
+        return e.value
+
+    def left_mode(self, on: bool):
+        self._check(self.lib.ccq_hip_left_mode(self.h, int(on)), "left_mode")
+
+    def set_l(self, l_vec: np.ndarray):
+        x = np.ascontiguousarray(l_vec, dtype=np.float64)
+        self._check(self.lib.ccq_hip_set_l(self.h, _ptr(x), x.size), "set_l")
+
+    def get_l(self) -> np.ndarray:
+        out = np.empty(self.l_size)
+        self._check(self.lib.ccq_hip_get_l(self.h, _ptr(out), out.size), "get_l")
+        return out
+
+    def cct3(self):
+        """crcc23's (e23a, e23b, e23c, e23d)."""
+        e = (ctypes.c_double * 4)()
+        self._check(self.lib.ccq_hip_cct3(self.h, e), "cct3")
+        return tuple(e)
 
     def profile(self, enable: bool):
         self._check(self.lib.ccq_hip_profile(self.h, int(enable)), "profile")

@@ -22,6 +22,18 @@ AMP_BLOCKS = {
     ('T4', (1, 1, 1, 1, 1, 1, 1, 1)): 't4bbbb',
 }
 RES_OF = {v: ('R' + k[0][1:], k[1]) for k, v in AMP_BLOCKS.items()}
+# left-CC amplitudes and residuals, CC(t;3) L3 / M3 accumulators
+AMP_BLOCKS.update({
+    ('L1', (0, 0)): 'l1a', ('L1', (1, 1)): 'l1b',
+    ('L2', (0, 0, 0, 0)): 'l2aa', ('L2', (0, 1, 0, 1)): 'l2ab', ('L2', (1, 1, 1, 1)): 'l2bb',
+})
+for _k, _v in list(AMP_BLOCKS.items()):
+    if _k[0] in ('L1', 'L2'):
+        RES_OF[_v] = ('LR' + _k[0][1:], _k[1])
+for _sp, _nm in (((0, 0, 0, 0, 0, 0), 'aaa'), ((0, 0, 1, 0, 0, 1), 'aab'),
+                 ((0, 1, 1, 0, 1, 1), 'abb'), ((1, 1, 1, 1, 1, 1), 'bbb')):
+    RES_OF['l3' + _nm] = ('L3', _sp)
+    RES_OF['m3' + _nm] = ('M3', _sp)
 
 
 class Dims:

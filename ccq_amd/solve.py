@@ -101,3 +101,72 @@ def summary(e_ref: float, e_cor: float, calc_type: str) -> str:
              f"{'Correlation':>20}{e_cor:18.12f}",
              f"{calc_type:>20}{e_ref + e_cor:18.12f}"]
     return "\n".join(lines)
+
+
+class _LeftIteration:
+    """jacobi_iter's five calls for the left-CC equations: solve_lcc binds
+    update_ptr => update_l2, calculate_energy_ptr => calculate_left_energy and
+    points the DIIS ring at l_vec (src/solve_cc.f90:101-173)."""
+
+    def __init__(self, eng: Engine):
+        self.eng = eng
+
+    def update(self):
+        self.eng.left_update()
+
+    def energy(self) -> float:
+        return self.eng.left_energy()
+
+    def diis_record(self, it: int):
+        self.eng.diis_record(it)
+
+    def residuum(self, it: int) -> float:
+        return self.eng.residuum(it)
+
+    def diis_extrapolate(self):
+        self.eng.diis_extrapolate()
+
+
+def solve_lcc(eng: Engine, e_cor: float, tol: float, max_iter: int, diis_space: int,
+              out=sys.stdout, stop_file: str = "cc.comm") -> SolveResult:
+    """Left-CCSD on the resident H-bar (ccq_hip_hbar first), l_vec from zero."""
+    if out:
+        print("Left Coupled cluster\n--------------------", file=out)
+        print(iter_header(), file=out)
+    eng.left_init(e_cor)
+    eng.left_mode(True)
+    try:
+        return jacobi_iter(_LeftIteration(eng), tol, max_iter, diis_space, out=out,
+                           stop_file=stop_file)
+    finally:
+        eng.left_mode(False)
+
+
+@dataclasses.dataclass
+class CCT3Result:
+    ccsdt: SolveResult
+    left: SolveResult
+    e23: tuple          # crcc23's (A, B, C, D) corrections
+
+    @property
+    def en_cor(self) -> float:
+        return self.ccsdt.en_cor + self.e23[3]
+
+
+def cct3(eng: Engine, tol: float, max_iter: int, diis_space: int, out=sys.stdout) -> CCT3Result:
+    """CC(t;3): CCSDt, H-bar, left-CCSD, crcc23 (src/calc_driver.f90:53-72)."""
+    r = jacobi_iter(eng, tol, max_iter, diis_space, out=out)
+    eng.hbar()
+    left = solve_lcc(eng, r.en_cor, tol, max_iter, diis_space, out=out)
+    e23 = eng.cct3()
+    return CCT3Result(ccsdt=r, left=left, e23=e23)
+
+
+def print_cct3(e_ref: float, e_cor: float, e23) -> str:
+    """print_cct3 (src/printing.F90:234-258)."""
+    rows = [("CCSDt", 0.0), ("CC(t;3),A", e23[0]), ("CC(t;3),B", e23[1]),
+            ("CC(t;3),C", e23[2]), ("CC(t;3)", e23[3])]
+    lines = ["", f"{'Method':>20}{'Correlation (Eh)':>18}{'Total Energy (Eh)':>18}", " " * 10 + "-" * 46]
+    for name, de in rows:
+        lines.append(f"{name:>20}{e_cor + de:18.12f}{e_ref + e_cor + de:18.12f}")
+    return "\n".join(lines)

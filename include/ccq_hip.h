@@ -45,7 +45,10 @@ typedef struct {
 typedef struct {
   int64_t level;       /* 2 CCSD, 3 CCSDT, 4 CCSDTQ (run%lvl_t, run%lvl_q),
                           5 CCSDt: active-space triples (calc_type=CCSDt,
-                          run%act_ind_t = 1, src/parser.f90:380-390)       */
+                          run%act_ind_t = 1, src/parser.f90:380-390),
+                          6 CC(t;3): CCSDt plus the H-bar / left-CCSD /
+                          crcc23 tail (run%hbar, run%lcc, run%mm_23,
+                          src/calc_driver.f90:53-72)                       */
   int64_t rhf;         /* run%rhf: closed-shell copies T2C=T2A, T1B=T1A ...
                           (ignored by CCSDt, as update_clusters_t3_opt does) */
   double shift;        /* run%shift: denominator shift                     */
@@ -98,6 +101,39 @@ int ccq_hip_residuum(ccq_hip_ctx* ctx, int64_t iter, double* res);
 
 /* DIIS extrapolation over the ring: calc_diis (src/diis.F90:11-84). */
 int ccq_hip_diis_extrapolate(ccq_hip_ctx* ctx);
+
+/* ---- CC(t;3) tail (level 6), after the CCSDt iteration has converged ---- */
+
+/* One- and two-body H-bar from the resident T1, T2: replaces hbar2
+ * (src/hbar.f90:10-720) plus add_twobody_hbar (:722-745). */
+int ccq_hip_hbar(ccq_hip_ctx* ctx);
+
+/* Start the left-CCSD solve: zero l_vec and fix E_cor for the denominators
+ * (the set-up of solve_lcc, src/solve_cc.f90:101-173; l_vec allocated in
+ * init_vecs, src/calc_driver.f90:229-233). */
+int ccq_hip_left_init(ccq_hip_ctx* ctx, double e_cor);
+
+/* One left-CCSD Jacobi step: replaces update_ptr => update_l2
+ * (src/update_lcc.f90:10-104, left_jacobi_update :208-280). */
+int ccq_hip_left_update(ccq_hip_ctx* ctx);
+
+/* |LH| + E_cor of the last step: replaces calculate_energy_ptr =>
+ * calculate_left_energy (src/energy.f90:103-136). */
+int ccq_hip_left_energy(ccq_hip_ctx* ctx, double* e);
+
+/* on != 0: ccq_hip_diis_record / residuum / diis_extrapolate act on l_vec
+ * (conv%vec_ptr => cc%l_vec, conv%vec_size = cc%l_size in solve_lcc). */
+int ccq_hip_left_mode(ccq_hip_ctx* ctx, int64_t on);
+
+/* cc%l_size = pos(6)-1 (T1/T2 layout of t_vec) and copies of cc%l_vec. */
+int64_t ccq_hip_l_size(const ccq_hip_ctx* ctx);
+int ccq_hip_set_l(ccq_hip_ctx* ctx, const double* l_vec, int64_t l_size);
+int ccq_hip_get_l(ccq_hip_ctx* ctx, double* l_vec, int64_t l_size);
+
+/* Triples correction over the triples outside the P space with the four
+ * denominators A-D: replaces crcc23 (src/mm_correct.f90:11-343).
+ * e23[0..3] = cc%mm_en_cor_a .. _d (CC(t;3) energy = E_cor + e23[3]). */
+int ccq_hip_cct3(ccq_hip_ctx* ctx, double* e23);
 
 /* Optional per-operation timing with HIP events on the context's stream.
  * ccq has no counterpart (it prints only wall time per iteration). */

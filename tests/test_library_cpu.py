@@ -32,8 +32,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_embedded_ir_is_current():
-    gh = emit.generator_hash()
     for method in emit.METHODS:
+        gh = emit.generator_hash(method)
         for rhf in emit.VARIANTS[method]:
             tag = "rhf" if rhf else "uhf"
             path = os.path.join(ROOT, "ccq_amd", "csrc", f"ir_{emit.STEM[method]}_{tag}.inc")
