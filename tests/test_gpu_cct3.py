@@ -122,8 +122,9 @@ def test_left_solve_and_correction_match_oracle(oa, ob):
     eng.set_l(g['l_vec'])
     e23 = eng.cct3()
     assert np.abs(np.array(e23) - g['e23']).max() < 1e-12
-    # T is untouched by the tail
-    np.testing.assert_allclose(eng.get_t(), g['tvec'], atol=0, rtol=0)
+    # T is untouched by the tail (the stored vector is antisymmetric to roundoff,
+    # so a pack / unpack round trip may move an image by ~1e-20)
+    np.testing.assert_allclose(eng.get_t(), g['tvec'], atol=1e-15, rtol=0)
     eng.close()
 
 
