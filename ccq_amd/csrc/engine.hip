@@ -947,7 +947,7 @@ struct ccq_hip_ctx {
     ccqk::UpdateParams P = update_params(u);
     const Tensor& R = tensors[u.resid];
     if (u.left) {
-      // update_l2 (src/update_lcc.f90:206-273): L -= LH / (Hbar_aa - Hbar_ii - E_cor + shift)
+      // left_jacobi_update (src/update_lcc.f90:208-280): L -= LH / (Hbar_aa - Hbar_ii - E_cor + shift)
       P.shift = shift - left_ecor;
       double* l = lvec.p + blk_off.at(dimkey(u.block));
       HIPCHK(ccqk::launch_update(stream, P, l, R.ptr, itab + u.dev_perm_off, dtab + u.dev_sign_off,

@@ -6,7 +6,9 @@
 ! (src/integrals.f90:212-364, src/energy.f90:49-101), then runs ccq's
 ! jacobi_iter (src/solve_cc.f90:175-269) with every amplitude operation on
 ! the GPU through ccq_hip_mod, printing ccq's iteration and summary lines
-! (src/printing.F90:303-349, 260-284).
+! (src/printing.F90:303-349, 260-284).  calc_type=CCT3 adds the CC(t;3) tail
+! of calc_driver (src/calc_driver.f90:53-72): H-bar, the left-CCSD
+! iteration through the same jacobi_iter, crcc23 and print_cct3.
 program ccq_drv
 
     use, intrinsic :: iso_c_binding
@@ -21,13 +23,12 @@ program ccq_drv
     real(dp) :: tol, shift
     real(dp), allocatable :: e1int(:,:), e2int(:,:,:,:), f_a(:,:), f_b(:,:)
     real(dp), allocatable :: v_aa(:,:,:,:), v_ab(:,:,:,:)
-    real(dp) :: en_repul, en_ref, e_cor_new, e_cor, diff, res, conv(3)
-    integer(c_int64_t) :: it, i, j, k, a, b, ios, unit_in, itol, eq
-    logical :: rhf, converged
+    real(dp) :: en_repul, en_ref, e_cor, e_left, e23(4), t0
+    integer(c_int64_t) :: i, j, k, a, b, ios, unit_in, itol, eq
+    logical :: rhf, converged, cct3
     type(ccq_hip_dims) :: dims
     type(ccq_hip_opts) :: opts
     type(c_ptr) :: ctx
-    real(dp) :: t0, t1
 
     call get_command_argument(1, inp)
     if (len_trim(inp) == 0) then
@@ -37,6 +38,7 @@ program ccq_drv
 
     ! defaults (src/parser.f90:779-844)
     froz_spin = 0; nel = -1; nvirt = -1; mult = 1; act_occ = -1; act_unocc = -1
+    cct3 = .false.
     tol = 1.0e-4_dp; diis_space = 5; max_iter = 60; shift = 0.0_dp; rhf = .false.
     onebody = 'onebody.inp'; twobody = 'twobody.inp'; calc_type = 'CCSD'
 
@@ -84,8 +86,13 @@ program ccq_drv
     case ('CCSD'); level = 2
     case ('CCSDT'); level = 3
     case ('CCSDTQ'); level = 4
-    case ('CCSDt')
+    case ('CCSDt', 'CCT3', 'CCt3', 'CC(t;3)')
         level = 5
+        cct3 = trim(calc_type) /= 'CCSDt'
+        if (cct3) then
+            level = 6
+            calc_type = 'CC(t;3)'
+        end if
         ! src/parser.f90:251-254
         if (act_occ == -1 .and. act_unocc == -1) then
             write(0, '(a)') 'CONFIGURATION ERROR: Active space methods require an active space'
@@ -183,38 +190,28 @@ program ccq_drv
 
     write(*, '(a)') 'Coupled cluster'
     write(*, '(a)') '---------------'
-    write(*, '(/2x,a4,3(a15),a16)') 'It.', 'E (Corr)', 'dE', 'Residuum', 'Wall Time'
-    write(*, '(2x,65("-"))')
-    conv = 1.0_dp
-    e_cor = 0.0_dp
-    converged = .false.
-    call cpu_time(t0)
-    do it = 1, max_iter
-        call hip_check(ctx, ccq_hip_update(ctx), 'update')
-        call hip_check(ctx, ccq_hip_energy(ctx, e_cor_new), 'energy')
-        diff = e_cor - e_cor_new
-        conv(1) = conv(2); conv(2) = conv(3); conv(3) = diff
-        e_cor = e_cor_new
-        call hip_check(ctx, ccq_hip_diis_record(ctx, it), 'diis_record')
-        call hip_check(ctx, ccq_hip_residuum(ctx, it, res), 'residuum')
-        if (mod(it, diis_space + 1) == 0) then
-            write(*, '(a)') '      DIIS cycle'
-            call hip_check(ctx, ccq_hip_diis_extrapolate(ctx), 'calc_diis')
-        end if
-        call cpu_time(t1)
-        write(*, '(2x,i4,3(f15.10),i5,'' min'',f5.1,'' s'')') it, e_cor_new, diff, res, 0, t1 - t0
-        t0 = t1
-        if (all(abs(conv) < tol)) then
-            converged = .true.
-            exit
-        end if
-        if (e_cor_new /= e_cor_new) then
-            write(*, '(a)') 'FAILED TO CONVERGE'
-            exit
-        end if
-        if (it == max_iter) write(*, '(a)') 'FAILED TO CONVERGE IN', max_iter, &
-            ' ITERATIONS. KEEPING LAST T VECTOR.'
-    end do
+    call jacobi_iter(.false., e_cor, converged)
+
+    if (cct3) then
+        ! hbar2, then solve_lcc with update_l2 / calculate_left_energy
+        call hip_check(ctx, ccq_hip_hbar(ctx), 'hbar2')
+        write(*, '(a)') 'Left Coupled cluster'
+        write(*, '(a)') '--------------------'
+        call hip_check(ctx, ccq_hip_left_init(ctx, e_cor), 'solve_lcc')
+        call hip_check(ctx, ccq_hip_left_mode(ctx, 1_c_int64_t), 'solve_lcc')
+        call jacobi_iter(.true., e_left, converged)
+        call hip_check(ctx, ccq_hip_left_mode(ctx, 0_c_int64_t), 'solve_lcc')
+        ! crcc23 and print_cct3 (src/printing.F90:234-258)
+        call hip_check(ctx, ccq_hip_cct3(ctx, e23), 'crcc23')
+        write(*, '(/a20,2a18)') 'Method', 'Correlation (Eh)', 'Total Energy (Eh)'
+        write(*, '(10x,46("-"))')
+        write(*, '(a20,2f18.12)') 'CCSDt', e_cor, en_ref + e_cor
+        write(*, '(a20,2f18.12)') 'CC(t;3),A', e_cor + e23(1), en_ref + e_cor + e23(1)
+        write(*, '(a20,2f18.12)') 'CC(t;3),B', e_cor + e23(2), en_ref + e_cor + e23(2)
+        write(*, '(a20,2f18.12)') 'CC(t;3),C', e_cor + e23(3), en_ref + e_cor + e23(3)
+        write(*, '(a20,2f18.12)') 'CC(t;3)', e_cor + e23(4), en_ref + e_cor + e23(4)
+        e_cor = e_cor + e23(4)
+    end if
 
     write(*, '(/a)') 'Calculation Summary (Eh)'
     write(*, '(a)') '------------------------'
@@ -222,5 +219,55 @@ program ccq_drv
     write(*, '(a20,f18.12)') 'Correlation', e_cor
     write(*, '(a20,f18.12)') trim(calc_type), en_ref + e_cor
     i = ccq_hip_destroy(ctx)
+
+contains
+
+    ! jacobi_iter (src/solve_cc.f90:175-269) over the engine: the T equations,
+    ! or with left the left-CC equations (update_l2 / calculate_left_energy)
+    subroutine jacobi_iter(left, e_out, converged)
+        logical, intent(in) :: left
+        real(dp), intent(out) :: e_out
+        logical, intent(out) :: converged
+        real(dp) :: e_new, e_old, diff, res, conv(3), t0, t1
+        integer(c_int64_t) :: it
+        write(*, '(/2x,a4,3(a15),a16)') 'It.', 'E (Corr)', 'dE', 'Residuum', 'Wall Time'
+        write(*, '(2x,65("-"))')
+        conv = 1.0_dp
+        e_old = 0.0_dp
+        converged = .false.
+        call cpu_time(t0)
+        do it = 1, max_iter
+            if (left) then
+                call hip_check(ctx, ccq_hip_left_update(ctx), 'update_l2')
+                call hip_check(ctx, ccq_hip_left_energy(ctx, e_new), 'calculate_left_energy')
+            else
+                call hip_check(ctx, ccq_hip_update(ctx), 'update')
+                call hip_check(ctx, ccq_hip_energy(ctx, e_new), 'energy')
+            end if
+            diff = e_old - e_new
+            conv(1) = conv(2); conv(2) = conv(3); conv(3) = diff
+            e_old = e_new
+            call hip_check(ctx, ccq_hip_diis_record(ctx, it), 'diis_record')
+            call hip_check(ctx, ccq_hip_residuum(ctx, it, res), 'residuum')
+            if (mod(it, diis_space + 1) == 0) then
+                write(*, '(a)') '      DIIS cycle'
+                call hip_check(ctx, ccq_hip_diis_extrapolate(ctx), 'calc_diis')
+            end if
+            call cpu_time(t1)
+            write(*, '(2x,i4,3(f15.10),i5,'' min'',f5.1,'' s'')') it, e_new, diff, res, 0, t1 - t0
+            t0 = t1
+            if (all(abs(conv) < tol)) then
+                converged = .true.
+                exit
+            end if
+            if (e_new /= e_new) then
+                write(*, '(a)') 'FAILED TO CONVERGE'
+                exit
+            end if
+            if (it == max_iter) write(*, '(a)') 'FAILED TO CONVERGE IN', max_iter, &
+                ' ITERATIONS. KEEPING LAST T VECTOR.'
+        end do
+        e_out = e_old
+    end subroutine jacobi_iter
 
 end program ccq_drv

@@ -105,6 +105,62 @@ module ccq_hip_mod
             type(c_ptr), value :: ctx
         end function ccq_hip_diis_extrapolate
 
+        ! CC(t;3) tail: hbar2 (src/hbar.f90:10-720), solve_lcc / update_l2
+        ! (src/solve_cc.f90:101-173, src/update_lcc.f90:10-280) and crcc23
+        ! (src/mm_correct.f90:11-343)
+        integer(c_int) function ccq_hip_hbar(ctx) bind(c, name='ccq_hip_hbar')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function ccq_hip_hbar
+
+        integer(c_int) function ccq_hip_left_init(ctx, e_cor) bind(c, name='ccq_hip_left_init')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: e_cor
+        end function ccq_hip_left_init
+
+        integer(c_int) function ccq_hip_left_update(ctx) bind(c, name='ccq_hip_left_update')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function ccq_hip_left_update
+
+        integer(c_int) function ccq_hip_left_energy(ctx, e) bind(c, name='ccq_hip_left_energy')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: e
+        end function ccq_hip_left_energy
+
+        integer(c_int) function ccq_hip_left_mode(ctx, on) bind(c, name='ccq_hip_left_mode')
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), value :: on
+        end function ccq_hip_left_mode
+
+        integer(c_int64_t) function ccq_hip_l_size(ctx) bind(c, name='ccq_hip_l_size')
+            import :: c_int64_t, c_ptr
+            type(c_ptr), value :: ctx
+        end function ccq_hip_l_size
+
+        integer(c_int) function ccq_hip_set_l(ctx, l_vec, l_size) bind(c, name='ccq_hip_set_l')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: l_vec(*)
+            integer(c_int64_t), value :: l_size
+        end function ccq_hip_set_l
+
+        integer(c_int) function ccq_hip_get_l(ctx, l_vec, l_size) bind(c, name='ccq_hip_get_l')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: l_vec(*)
+            integer(c_int64_t), value :: l_size
+        end function ccq_hip_get_l
+
+        integer(c_int) function ccq_hip_cct3(ctx, e23) bind(c, name='ccq_hip_cct3')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: e23(4)
+        end function ccq_hip_cct3
+
         type(c_ptr) function ccq_hip_last_error(ctx) bind(c, name='ccq_hip_last_error')
             import :: c_ptr
             type(c_ptr), value :: ctx

@@ -155,9 +155,10 @@ def left_denominators(H: Hbar):
 
 
 def left_jacobi(H: Hbar, t2, l1, l2, ecor, shift=0.0):
-    """One update_l2 step (src/update_lcc.f90:206-273): all residuals from the
-    same L, then L -= LH / (Hbar_aa - Hbar_ii - E_cor + shift).  Returns the
-    residual (ccq's lh_vec) of the step."""
+    """One update_l2 step (src/update_lcc.f90:10-104, left_jacobi_update
+    :208-280): all residuals from the same L, then
+    L -= LH / (Hbar_aa - Hbar_ii - E_cor + shift).  Returns the residual
+    (ccq's lh_vec) of the step."""
     r1, r2 = lambda_residual(H, t2, l1, l2)
     ho, hv = left_denominators(H)
     d1 = hv[None, :] - ho[:, None] - ecor + shift

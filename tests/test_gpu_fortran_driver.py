@@ -43,3 +43,31 @@ def test_driver_h4_ccsdt(tmp_path):
     # published value, so compare the correlation energy
     assert abs(ref - kat["reference_energy"]) < 5e-10
     assert abs((tot - ref) - (kat["CCSDT_total"] - kat["reference_energy"])) < 1e-10
+
+
+@pytest.mark.skipif(not os.path.exists(DRV), reason="fortran/ccq_drv not built (make -C fortran)")
+def test_driver_cct3_matches_oracle(tmp_path):
+    """calc_type=CCT3 through the driver: CCSDt, H-bar, left-CCSD, crcc23 and
+    ccq's print_cct3 table, against the oracle's energies stored in
+    tests/golden/cct3_n7_33.npz (same seeded 7-orbital system)."""
+    import numpy as np
+    from ccq_amd import synth
+    print("case fortran driver cct3 n7", flush=True)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "cct3_n7_33.npz"))
+    s = synth.make_system(7, 3, 20, 11)
+    inp = synth.write_inputs(str(tmp_path), s, "CCT3", act_occ=1, act_unocc=1, tol="1.0e-11",
+                             diis_space=5, max_iterations=80)
+    out = subprocess.check_output([DRV, inp], timeout=120).decode()
+    ref, tot = run_ccq("CC(t;3)", inp)
+    ecor, e23 = float(g["ecor"]), g["e23"]
+    assert abs((tot - ref) - (ecor + e23[3])) < 1e-9
+    rows = {}
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] in ("CCSDt", "CC(t;3),A", "CC(t;3),B", "CC(t;3),C",
+                                            "CC(t;3)"):
+            rows[parts[0]] = float(parts[1])
+    assert abs(rows["CCSDt"] - ecor) < 1e-9
+    for name, k in (("CC(t;3),A", 0), ("CC(t;3),B", 1), ("CC(t;3),C", 2), ("CC(t;3)", 3)):
+        assert abs(rows[name] - (ecor + e23[k])) < 1e-9, name
+    assert "Left Coupled cluster" in out
