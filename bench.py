@@ -1,17 +1,24 @@
-"""Benchmark: time-to-|dE|<1e-7 Eh of the CC iteration on one MI355X.
+"""Benchmark: time-to-|dE|<1e-7 Eh of the CC iteration on one MI355X,
+mixed precision against plain FP64.
 
 Workload (N=1): closed-shell CCSD, 100 spatial orbitals (10 occupied,
 90 virtual), synthetic integrals (pq|rs) = sum_L B B with L = 200 from seed 1,
-tol-driven ccq iteration (diis_space 5, shift 0), plain FP64.
+tol-driven ccq iteration (diis_space 5, shift 0).
 
 Protocol
   1. build the system and upload it (untimed);
-  2. W warm-up iterations from T = 0, then a tight untimed solve to get the
-     converged FP64 energy E_conv (tol 1e-11);
-  3. reset T = 0 and time exactly K iterations (update, energy, DIIS ring,
-     residuum, DIIS every 6th), synchronising the stream around the timed
-     region and after every iteration (the energy is read back each time);
-  4. value = wall time until the first iteration with |E - E_conv| < 1e-7.
+  2. W warm-up iterations from T = 0 in each precision, then a tight untimed
+     FP64 solve to get the converged energy E_conv and amplitudes (tol 1e-11);
+  3. plain leg: reset T = 0 and time exactly K FP64 iterations (update,
+     energy, DIIS ring, residuum, DIIS every 6th), synchronising the stream
+     around the timed region and after every iteration;
+  4. mixed leg: the same K iterations with FP32-MFMA contractions until
+     |dE| < 1e-6, FP64 after that (the FP64 refinement);
+  5. value = mixed leg's wall time until the first iteration with
+     |E - E_conv| < 1e-7; the plain leg's time, the speedup, and the mixed
+     leg's final energy and max-abs T2 error against the FP64 solution go
+     into config.  (The learned starting guess is not built; both legs start
+     from T = 0.)
   Roofline: the dominant GEMM class (the contraction shape with the largest
   share of the sweep) timed with HIP events on the engine's stream during one
   extra profiled iteration; algorithmic FLOPs 2*M*N*K per launch.
@@ -43,6 +50,7 @@ from ccq_amd.inputs import build_integrals  # noqa: E402
 METRIC = "time-to-|dE|<1e-7 Eh, mixed+learned vs plain FP64 (1 GPU); % FP64 MFMA peak"
 PEAK_FP64_MFMA = 78.6e12
 PEAK_HBM = 8.0e12
+SWITCH_DE = 1e-6     # mixed leg: FP32 contractions until |dE| falls below this
 
 
 def log(msg):
@@ -55,11 +63,19 @@ def make_workload(name: str):
     return s, ints
 
 
-def run_iterations(eng, nsteps, diis_space, energies=None, stamps=None):
+def run_iterations(eng, nsteps, diis_space, energies=None, stamps=None, switch_de=None):
+    """nsteps iterations of ccq's loop.  switch_de: mixed precision -- FP32
+    contractions until |dE| < switch_de, then FP64 (solve.Mixed's schedule)."""
     en_cor = 0.0
+    low = switch_de is not None
+    if low:
+        eng.set_precision(1)
     for it in range(1, nsteps + 1):
         eng.update()
         e = eng.energy()
+        if low and abs(en_cor - e) < switch_de:
+            low = False
+            eng.set_precision(0)
         eng.diis_record(it)
         eng.residuum(it)
         if it % (diis_space + 1) == 0:
@@ -70,6 +86,7 @@ def run_iterations(eng, nsteps, diis_space, energies=None, stamps=None):
             eng.sync()
             stamps.append(time.perf_counter())
         en_cor = e
+    eng.set_precision(0)
     return en_cor
 
 
@@ -171,25 +188,36 @@ def main():
     eng.set_ints(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb)
     log(f"upload {time.perf_counter() - t0:.1f}s")
     zero = np.zeros(eng.t_size)
-    # warm-up and the converged FP64 reference energy (untimed)
+    # warm-up (both precisions) and the converged FP64 reference (untimed)
     eng.set_t(zero)
     run_iterations(eng, args.warmup, diis)
     eng.set_t(zero)
-    e_conv, it_conv = converge(eng, diis)
-    log(f"E_conv = {e_conv:.12f} after {it_conv} iterations")
-    # timed region
+    run_iterations(eng, args.warmup, diis, switch_de=0.0)
     eng.set_t(zero)
-    energies, stamps = [], []
-    eng.sync()
-    start = time.perf_counter()
-    run_iterations(eng, args.steps, diis, energies, stamps)
-    eng.sync()
-    total = time.perf_counter() - start
-    hit = next((i for i, e in enumerate(energies) if abs(e - e_conv) < 1e-7), None)
-    value = (stamps[hit] - start) if hit is not None else None
-    n_needed = hit + 1 if hit is not None else None
-    log(f"timed {args.steps} iterations: {total * 1e3 / args.steps:.3f} ms/iter; "
-        f"|dE|<1e-7 at iteration {n_needed}: {value}")
+    e_conv, it_conv = converge(eng, diis)
+    t_conv = eng.get_t()
+    log(f"E_conv = {e_conv:.12f} after {it_conv} iterations")
+    legs = {}
+    for leg, switch in (("plain", None), ("mixed", SWITCH_DE)):
+        eng.set_t(zero)
+        energies, stamps = [], []
+        eng.sync()
+        start = time.perf_counter()
+        run_iterations(eng, args.steps, diis, energies, stamps, switch_de=switch)
+        eng.sync()
+        total = time.perf_counter() - start
+        hit = next((i for i, e in enumerate(energies) if abs(e - e_conv) < 1e-7), None)
+        t_end = eng.get_t()
+        legs[leg] = dict(total=total, value=(stamps[hit] - start) if hit is not None else None,
+                         n_needed=hit + 1 if hit is not None else None,
+                         e_err=abs(energies[-1] - e_conv),
+                         t2_err=float(np.abs(t_end - t_conv).max()))
+        log(f"{leg}: {args.steps} iterations {total * 1e3 / args.steps:.3f} ms/iter; "
+            f"|E-E_conv|<1e-7 at iteration {legs[leg]['n_needed']}: {legs[leg]['value']}; "
+            f"final |dE| {legs[leg]['e_err']:.2e}, max|dT| {legs[leg]['t2_err']:.2e}")
+    value = legs["mixed"]["value"]
+    n_needed = legs["plain"]["n_needed"]
+    total = legs["mixed"]["total"]
     # roofline of the dominant kernel: one profiled sweep (HIP events on the stream)
     eng.profile(True)
     eng.update()
@@ -226,14 +254,24 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64/f32 mixed",
         "data": "synthetic (seeded integrals, ccq onebody/twobody construction)",
-        "config": {"workload": f"{args.workload}: CCSD plain FP64, n={n} o={o} v={n - o}, "
+        "config": {"workload": f"{args.workload}: CCSD, n={n} o={o} v={n - o}, "
                                f"L={synth.WORKLOADS[args.workload]['nL']}, seed "
                                f"{synth.WORKLOADS[args.workload]['seed']}, diis 5, "
                                f"{'rhf' if args.rhf else 'uhf'} sweep",
-                   "iterations_to_1e-7": n_needed, "e_conv": e_conv,
-                   "mode": "plain FP64 (mixed precision and learned guess not built yet)"},
+                   "e_conv": e_conv,
+                   "mode": "mixed: FP32-MFMA tiled contractions until |dE|<1e-6, then FP64; "
+                           "starting guess T=0 (learned guess not built)",
+                   "plain_fp64": {"time_to_1e-7_s": legs["plain"]["value"],
+                                  "iterations_to_1e-7": legs["plain"]["n_needed"],
+                                  "ms_per_step": legs["plain"]["total"] * 1e3 / args.steps},
+                   "mixed": {"time_to_1e-7_s": legs["mixed"]["value"],
+                             "iterations_to_1e-7": legs["mixed"]["n_needed"],
+                             "final_energy_error": legs["mixed"]["e_err"],
+                             "final_max_abs_t_error": legs["mixed"]["t2_err"]},
+                   "speedup_vs_plain": (legs["plain"]["value"] / value
+                                        if value and legs["plain"]["value"] else None)},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
                      "traffic": measured_traffic(dom_name),

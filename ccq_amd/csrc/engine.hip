@@ -182,6 +182,9 @@ struct ccq_hip_ctx {
   std::map<std::string, int> tindex;
   std::vector<Section> sections;
   bool ints_ready = false;
+  // arithmetic of the LDS-tiled contractions of the sweep: 0 FP64 MFMA,
+  // 1 FP32 MFMA (mixed precision; everything else stays FP64)
+  int prec = 0;
   // CC(t;3) tail (level 6): H-bar, left-CCSD and the triples correction run
   // as programs of a second IR on the same resident tensors
   bool tail = false;
@@ -890,6 +893,7 @@ struct ccq_hip_ctx {
       g.alpha = op.coef;
       g.accumulate = acc;
       g.ws = ws.p;
+      g.prec = prec;
       HIPCHK(ccqk::launch_gemm(stream, g));
     } else {
       const bool aligned =
@@ -994,7 +998,8 @@ struct ccq_hip_ctx {
       run_program(prog);
       return;
     }
-    Graph& G = graphs[prog];
+    // one graph per program and arithmetic (the launches differ)
+    Graph& G = graphs[prog + (prec ? ":fp32" : "")];
     if (G.x && G.key != key) drop_graph(G);
     if (!G.x) {
       HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -1743,6 +1748,13 @@ int ccq_hip_get_l(ccq_hip_ctx* ctx, double* l_vec, int64_t l_size) {
 
 int ccq_hip_cct3(ccq_hip_ctx* ctx, double* e23) {
   return guarded(ctx, [&] { ctx->cct3(e23); });
+}
+
+int ccq_hip_set_precision(ccq_hip_ctx* ctx, int64_t mode) {
+  return guarded(ctx, [&] {
+    if (mode != 0 && mode != 1) throw Fail("precision mode must be 0 (FP64) or 1 (FP32 MFMA)");
+    ctx->prec = (int)mode;
+  });
 }
 
 int ccq_hip_profile(ccq_hip_ctx* ctx, int64_t enable) {

@@ -35,6 +35,9 @@ struct GemmArgs {
   int nsplit = 1;
   int64_t kchunk = 0;
   int group = 1;           // tile rows per group of the XCD-aware tile order
+  // arithmetic of the LDS-tiled kernel: 0 FP64 MFMA; 1 FP32 MFMA (operands
+  // rounded to FP32 when staged into LDS, FP32 accumulation, FP64 epilogue)
+  int prec = 0;
   // split-K workspace of at least gemm_ws_need doubles
   double* ws = nullptr;
 };

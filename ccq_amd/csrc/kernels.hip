@@ -1,10 +1,13 @@
 // CDNA4 (gfx950) kernels of the coupled-cluster amplitude engine.
 //
-//  * gemm_f64_tab   FP64 MFMA (v_mfma_f64_16x16x4f64) tile GEMM whose operand
-//                   loaders and epilogue address tensors through per-index
-//                   offset tables, so any index permutation / sub-block view
-//                   of a tensor is read and written in place (no separate
-//                   reorder/sum_stripe passes; cf. src/cc/reorder_stripe.f90).
+//  * gemm_tab       MFMA tile GEMM (FP64 v_mfma_f64_16x16x4f64, or FP32
+//                   v_mfma_f32_16x16x4f32 for the mixed-precision form) whose
+//                   operand loaders and epilogue address tensors through
+//                   per-index offset tables, so any index permutation /
+//                   sub-block view of a tensor is read and written in place
+//                   (no separate reorder/sum_stripe passes; cf.
+//                   src/cc/reorder_stripe.f90).
+//  * gemm_f64_reg   register-direct FP64 MFMA GEMM for skinny products.
 //  * perm_add       C(perm) = beta C + alpha A through the same tables.
 //  * jacobi_update  R = A_G(U) over a block's antisymmetry group, t -= R/(D+shift)
 //                   on canonical tuples, signed scatter to every image
@@ -22,6 +25,7 @@
 #include "kernels.h"
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 namespace ccqk {
 
@@ -89,14 +93,19 @@ __device__ __forceinline__ void store_partial(const GemmArgs& g, int z, int64_t 
   g.ws[((int64_t)z * g.M + m) * g.N + n] = v;
 }
 
-// LDS tile of an operand: R x BK values.  KF (k-fastest in global memory):
-// stored row-major [R][BK+1] so a 16-lane group writing consecutive k and
-// the MFMA fragment read (16 consecutive rows, one k) are both bank-conflict
-// free.  Otherwise stored k-major [BK][R+2].
-template <int R, bool KF, int BK>
+// LDS tile of an operand: R x BK values of type T.  KF (k-fastest in global
+// memory): stored row-major [R][BK+P] so a 16-lane group writing consecutive
+// k and the MFMA fragment read (16 consecutive rows, one k) are bank-conflict
+// free; otherwise stored k-major [BK][R+Q].  FP64: P = 1, Q = 2.  FP32 (four
+// k per fragment read by the four 16-lane groups): P = 4 makes the row pitch
+// 4 mod 16 words apart and Q = 16 puts the four k rows 16 banks apart, so the
+// 64 lanes of one fragment read hit 64 distinct banks.
+template <typename T, int R, bool KF, int BK>
 struct LdsTile {
-  static constexpr int size = KF ? R * (BK + 1) : BK * (R + 2);
-  __device__ static int at(int row, int k) { return KF ? row * (BK + 1) + k : k * (R + 2) + row; }
+  static constexpr int P = sizeof(T) == 8 ? 1 : 4;
+  static constexpr int Q = sizeof(T) == 8 ? 2 : 16;
+  static constexpr int size = KF ? R * (BK + P) : BK * (R + Q);
+  __device__ static int at(int row, int k) { return KF ? row * (BK + P) + k : k * (R + Q) + row; }
 };
 
 // Per-thread loader of one operand slab (R rows x BK k-values per slab).
@@ -136,27 +145,57 @@ struct Loader {
       }
     }
   }
-  __device__ void stash(double* lds, const double (&reg)[L]) const {
+  template <typename T>
+  __device__ void stash(T* lds, const double (&reg)[L]) const {
 #pragma unroll
-    for (int j = 0; j < L; ++j) lds[LdsTile<R, KF, BK>::at(rr[j], kk[j])] = reg[j];
+    for (int j = 0; j < L; ++j) lds[LdsTile<T, R, KF, BK>::at(rr[j], kk[j])] = (T)reg[j];
   }
 };
 
-// LDS-tiled GEMM C(M x N) (+)= alpha * A(M x K) B(K x N) on
-// v_mfma_f64_16x16x4f64, for products with both output extents large
-// (MFMA bound).  256 threads = 4 waves in a 2 x 2 grid; each wave owns a
-// (BM/2) x (BN/2) sub-tile of 16x16 MFMA tiles.  Operands are gathered through
-// int32 offset tables (rowA/kA, kB/colB); the next K slab is fetched into
-// registers while the current one is multiplied out of LDS.  blockIdx.z owns
-// K range [z*kchunk, (z+1)*kchunk) (split-K, completed by gemm_splitk_reduce).
-template <int BM, int BN, int BK, bool AKF, bool BKF>
-__global__ __launch_bounds__(256, (BM * BN >= 128 * 128) ? 1 : 2) void gemm_f64_tab(GemmArgs g) {
+// MFMA step and D-fragment row of one 16x16 tile, by arithmetic type.
+// v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 read the same A / B
+// fragments (lane l: row l%16, k = l/16); their D layouts differ (measured on
+// gfx950): FP64 lane l, register v holds row (l/16) + 4v, FP32 row 4(l/16) + v;
+// column l%16 in both.
+template <typename T>
+struct Mma;
+template <>
+struct Mma<double> {
+  typedef d4 acc_t;
+  __device__ static acc_t step(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct Mma<float> {
+  typedef f4 acc_t;
+  __device__ static acc_t step(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// LDS-tiled GEMM C(M x N) (+)= alpha * A(M x K) B(K x N), for products with
+// both output extents large (MFMA bound).  256 threads = 4 waves in a 2 x 2
+// grid; each wave owns a (BM/2) x (BN/2) sub-tile of 16x16 MFMA tiles.
+// Operands are gathered in FP64 through int32 offset tables (rowA/kA,
+// kB/colB) and staged into LDS as T; the next K slab is fetched into
+// registers while the current one is multiplied out of LDS.  T = double runs
+// on v_mfma_f64_16x16x4f64; T = float rounds the operands to FP32 and
+// accumulates in FP32 on v_mfma_f32_16x16x4f32 (the mixed-precision form).
+// blockIdx.z owns K range [z*kchunk, (z+1)*kchunk) (split-K, completed by
+// gemm_splitk_reduce).
+template <typename T, int BM, int BN, int BK, bool AKF, bool BKF>
+__global__ __launch_bounds__(256, (sizeof(T) == 8 && BM * BN >= 128 * 128) ? 1 : 2) void gemm_tab(
+    GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
-  using TA = LdsTile<BM, AKF, BK>;
-  using TB = LdsTile<BN, BKF, BK>;
-  __shared__ double As[TA::size];
-  __shared__ double Bs[TB::size];
+  using TA = LdsTile<T, BM, AKF, BK>;
+  using TB = LdsTile<T, BN, BKF, BK>;
+  using MM = Mma<T>;
+  __shared__ T As[TA::size];
+  __shared__ T Bs[TB::size];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -175,11 +214,11 @@ __global__ __launch_bounds__(256, (BM * BN >= 128 * 128) ? 1 : 2) void gemm_f64_
   lb.init(g.colB, n0, g.N, tid);
   double ra[Loader<BM, AKF, BK>::L], rb[Loader<BN, BKF, BK>::L];
 
-  d4 acc[TM][TN];
+  typename MM::acc_t acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+    for (int j = 0; j < TN; ++j) acc[i][j] = typename MM::acc_t{0, 0, 0, 0};
 
   const int64_t nsteps = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   if (nsteps > 0) {
@@ -199,7 +238,7 @@ __global__ __launch_bounds__(256, (BM * BN >= 128 * 128) ? 1 : 2) void gemm_f64_
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4);
       const int rr = lane & 15;
-      double af[TM], bf[TN];
+      T af[TM], bf[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) af[i] = As[TA::at(wm + 16 * i + rr, kr)];
 #pragma unroll
@@ -207,13 +246,10 @@ __global__ __launch_bounds__(256, (BM * BN >= 128 * 128) ? 1 : 2) void gemm_f64_
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < TN; ++j) acc[i][j] = MM::step(af[i], bf[j], acc[i][j]);
     }
   }
 
-  // D layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l, register v
-  // holds row (l/16) + 4*v, column l%16
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     const int64_t gn = n0 + wn + 16 * j + (lane & 15);
@@ -222,10 +258,11 @@ __global__ __launch_bounds__(256, (BM * BN >= 128 * 128) ? 1 : 2) void gemm_f64_
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t gm = m0 + wm + 16 * i + (lane >> 4) + 4 * v;
+        const int64_t gm = m0 + wm + 16 * i + MM::row(lane, v);
         if (gm >= g.M) continue;
-        if (g.nsplit == 1) store_c(g, gm, gn, acc[i][j][v]);
-        else store_partial(g, tz, gm, gn, acc[i][j][v]);
+        const double x = (double)acc[i][j][v];
+        if (g.nsplit == 1) store_c(g, gm, gn, x);
+        else store_partial(g, tz, gm, gn, x);
       }
   }
 }
@@ -959,16 +996,22 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
   g.group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, (int64_t)std::lround(std::sqrt(per_xcd))));
 }
 
+template <typename T, int BM, int BN, int KS>
+static void launch_tile_t(hipStream_t s, dim3 grid, const GemmArgs& g) {
+  if (g.a_kfast && g.b_kfast)
+    hipLaunchKernelGGL((gemm_tab<T, BM, BN, KS, true, true>), grid, dim3(256), 0, s, g);
+  else if (g.a_kfast)
+    hipLaunchKernelGGL((gemm_tab<T, BM, BN, KS, true, false>), grid, dim3(256), 0, s, g);
+  else if (g.b_kfast)
+    hipLaunchKernelGGL((gemm_tab<T, BM, BN, KS, false, true>), grid, dim3(256), 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_tab<T, BM, BN, KS, false, false>), grid, dim3(256), 0, s, g);
+}
+
 template <int BM, int BN, int KS>
 static void launch_tile(hipStream_t s, dim3 grid, const GemmArgs& g) {
-  if (g.a_kfast && g.b_kfast)
-    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, KS, true, true>), grid, dim3(256), 0, s, g);
-  else if (g.a_kfast)
-    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, KS, true, false>), grid, dim3(256), 0, s, g);
-  else if (g.b_kfast)
-    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, KS, false, true>), grid, dim3(256), 0, s, g);
-  else
-    hipLaunchKernelGGL((gemm_f64_tab<BM, BN, KS, false, false>), grid, dim3(256), 0, s, g);
+  if (g.prec == 1) launch_tile_t<float, BM, BN, KS>(s, grid, g);
+  else launch_tile_t<double, BM, BN, KS>(s, grid, g);
 }
 
 template <int RM, int RN, int WK>

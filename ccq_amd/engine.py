@@ -63,6 +63,7 @@ EXPORTS = {
     "ccq_hip_set_l": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_get_l": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_cct3": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_profile_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "ccq_hip_debug_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -160,6 +161,10 @@ class Engine:
 
     def diis_extrapolate(self):
         self._check(self.lib.ccq_hip_diis_extrapolate(self.h), "diis_extrapolate")
+
+    def set_precision(self, mode: int):
+        """0: FP64 MFMA contractions; 1: FP32 MFMA for the tiled contractions."""
+        self._check(self.lib.ccq_hip_set_precision(self.h, int(mode)), "set_precision")
 
     # CC(t;3) tail (calc_type "CC(t;3)")
 

@@ -25,7 +25,7 @@ class SolveResult:
     iterations: int
     converged: bool
     failed: bool
-    history: list      # (iter, e_cor, dE, residuum, seconds)
+    history: list      # (iter, e_cor, dE, residuum, seconds, FP32 contractions?)
 
 
 def fmt_iteration(it, e, de, res, secs):
@@ -48,8 +48,20 @@ def check_sig(path: str = "cc.comm") -> bool:
     return False
 
 
+@dataclasses.dataclass
+class Mixed:
+    """Mixed-precision schedule: sweeps run with FP32 MFMA contractions until
+    |dE| < switch_de, then FP64 to the end; with refine_every = k every k-th
+    sweep before the switch is FP64 as well.  Convergence (three consecutive
+    |dE| < tol) is only counted over FP64 sweeps, so the converged energy and
+    amplitudes are those of the FP64 equations."""
+    switch_de: float = 1e-6
+    refine_every: int = 0
+
+
 def jacobi_iter(eng: Engine, tol: float, max_iter: int, diis_space: int,
-                out=sys.stdout, on_iter=None, stop_file: str = "cc.comm") -> SolveResult:
+                out=sys.stdout, on_iter=None, stop_file: str = "cc.comm",
+                mixed: Mixed | None = None) -> SolveResult:
     conv = [1.0, 1.0, 1.0]
     en_cor = 0.0
     failed = False
@@ -57,7 +69,11 @@ def jacobi_iter(eng: Engine, tol: float, max_iter: int, diis_space: int,
     hist = []
     prev = time.perf_counter()
     it = 0
+    low = mixed is not None
     for it in range(1, max_iter + 1):
+        if mixed is not None:
+            fp32 = low and not (mixed.refine_every and it % mixed.refine_every == 0)
+            eng.set_precision(1 if fp32 else 0)
         eng.update()
         e_new = eng.energy()
         de = en_cor - e_new
@@ -70,7 +86,7 @@ def jacobi_iter(eng: Engine, tol: float, max_iter: int, diis_space: int,
                 print("      DIIS cycle", file=out)
             eng.diis_extrapolate()
         now = time.perf_counter()
-        hist.append((it, e_new, de, res, now - prev))
+        hist.append((it, e_new, de, res, now - prev, mixed is not None and fp32))
         if out:
             print(fmt_iteration(it, e_new, de, res, now - prev), file=out, flush=True)
         prev = now
@@ -80,6 +96,14 @@ def jacobi_iter(eng: Engine, tol: float, max_iter: int, diis_space: int,
             if out:
                 print("Signaled to end", file=out)
             break
+        if low and abs(de) < mixed.switch_de:
+            # final sweeps in FP64; only they count towards convergence
+            low = False
+            conv = [1.0, 1.0, 1.0]
+            if out:
+                print("      FP64 refinement", file=out)
+        elif mixed is not None and fp32:
+            conv = [1.0, 1.0, 1.0]
         if all(abs(c) < tol for c in conv):
             converged = True
             break
@@ -91,6 +115,8 @@ def jacobi_iter(eng: Engine, tol: float, max_iter: int, diis_space: int,
             if out:
                 print("FAILED TO CONVERGE IN", max_iter, " ITERATIONS. KEEPING LAST T VECTOR.", file=out)
             failed = True
+    if mixed is not None:
+        eng.set_precision(0)
     return SolveResult(en_cor=en_cor, iterations=it, converged=converged, failed=failed, history=hist)
 
 

@@ -135,6 +135,14 @@ int ccq_hip_get_l(ccq_hip_ctx* ctx, double* l_vec, int64_t l_size);
  * e23[0..3] = cc%mm_en_cor_a .. _d (CC(t;3) energy = E_cor + e23[3]). */
 int ccq_hip_cct3(ccq_hip_ctx* ctx, double* e23);
 
+/* Arithmetic of the bulk contractions of later ccq_hip_update calls: 0 FP64
+ * MFMA (default, ccq's precision); 1 FP32 MFMA for the LDS-tiled
+ * contractions (operands rounded to FP32 on the way into LDS, FP32
+ * accumulation, added into FP64 results).  Denominators, updates, energy,
+ * DIIS and the convergence test stay FP64; the caller refines with FP64
+ * sweeps (mode 0) before it stops.  No counterpart in ccq (FP64 only). */
+int ccq_hip_set_precision(ccq_hip_ctx* ctx, int64_t mode);
+
 /* Optional per-operation timing with HIP events on the context's stream.
  * ccq has no counterpart (it prints only wall time per iteration). */
 int ccq_hip_profile(ccq_hip_ctx* ctx, int64_t enable);

@@ -60,7 +60,7 @@ def test_full_ccsd_n24_parity():
     res = S.jacobi_iter(eng, tol=1e-9, max_iter=60, diis_space=5, out=None)
     assert res.converged and ref.converged
     assert res.iterations == ref.iterations
-    for (it, e, de, r, _), (it2, e2, de2, r2) in zip(res.history, ref.history):
+    for (it, e, de, r, *_), (it2, e2, de2, r2) in zip(res.history, ref.history):
         assert abs(e - e2) < 1e-9, (it, e, e2)
         assert abs(r - r2) < 1e-6
     assert abs(res.en_cor - ref.en_cor) < 1e-9

@@ -1,0 +1,129 @@
+"""Mixed precision: FP32-MFMA contractions with FP64 refinement
+(ccq_hip_set_precision, solve.Mixed), against the plain FP64 engine, which is
+itself pinned to the oracle by the other GPU tests.
+
+Bar: one FP32 sweep stays within 1e-5 of the FP64 sweep's change of the
+amplitudes (relative) and differs from it (the FP32 path ran); a mixed solve
+ends on the FP64 fixed point: energy within 1e-9 Eh and amplitudes within
+1e-8 of the plain FP64 solve; FP32 graph replays equal eager FP32 launches.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import solver as osolver
+from ccq_amd import engine as E
+from ccq_amd import solve as S
+from gpu_helpers import Case, antisym_random
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _random_state(c, method, seed, scale=0.05):
+    rng = np.random.default_rng(seed)
+    t = osolver.zero_amplitudes(method, c.ca, c.cb, c.va, c.vb)
+    lvl = {"CCSD": 2, "CCSDT": 3}[method]
+    for k in t:
+        if osolver.BLOCKS[k][0] <= lvl:
+            t[k] = antisym_random(rng, k, c.ca, c.cb, c.va, c.vb, scale)
+    return E.join_tvec(t, c.ca, c.cb, c.va, c.vb)
+
+
+def _engine(c, method, **kw):
+    eng = E.Engine(0, c.oa, c.ob, c.n, method, **kw)
+    eng.set_ints(c.fa, c.fb, c.vaa, c.vab, c.vbb)
+    return eng
+
+
+@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 24, 5, 5), ("CCSD", 30, 6, 4), ("CCSDT", 12, 3, 3)])
+def test_fp32_sweep_close_to_fp64(method, n, o, ob):
+    _say(f"case fp32 sweep {method} n={n} o={o} ob={ob}")
+    c = Case(n, o, 2 * n, 7, occ_b=ob)
+    vec = _random_state(c, method, 5)
+    out = []
+    for prec in (0, 1):
+        eng = _engine(c, method, shift=0.02)
+        eng.set_t(vec)
+        eng.set_precision(prec)
+        eng.update()
+        out.append(eng.get_t())
+        eng.close()
+    step = np.abs(out[0] - vec).max()
+    diff = np.abs(out[1] - out[0]).max()
+    assert diff > 1e-13, "the FP32 contractions did not run"
+    assert diff < 1e-5 * step, (diff, step)
+
+
+@pytest.mark.parametrize("method,n,o", [("CCSD", 24, 5), ("CCSDT", 12, 3)])
+def test_mixed_solve_reaches_fp64_fixed_point(method, n, o):
+    _say(f"case mixed solve {method} n={n} o={o}")
+    c = Case(n, o, 2 * n, 11)
+    plain = _engine(c, method)
+    r0 = S.jacobi_iter(plain, tol=1e-10, max_iter=80, diis_space=5, out=None)
+    mixed = _engine(c, method)
+    r1 = S.jacobi_iter(mixed, tol=1e-10, max_iter=80, diis_space=5, out=None,
+                       mixed=S.Mixed(switch_de=1e-6))
+    assert r0.converged and r1.converged
+    n32 = sum(1 for h in r1.history if h[5])
+    assert n32 >= 3, "no FP32 sweeps were run"
+    assert not r1.history[-1][5]
+    assert abs(r1.en_cor - r0.en_cor) < 1e-9
+    assert np.abs(mixed.get_t() - plain.get_t()).max() < 1e-8
+    plain.close()
+    mixed.close()
+
+
+def test_mixed_refine_every_schedule():
+    _say("case mixed refine_every")
+    c = Case(20, 4, 40, 13)
+    eng = _engine(c, "CCSD")
+    r = S.jacobi_iter(eng, tol=1e-10, max_iter=80, diis_space=5, out=None,
+                      mixed=S.Mixed(switch_de=1e-7, refine_every=3))
+    assert r.converged
+    flags = [h[5] for h in r.history]
+    # every third sweep before the switch is FP64
+    first64 = flags.index(False)
+    assert first64 == 2
+    ref = S.jacobi_iter(_engine(c, "CCSD"), tol=1e-10, max_iter=80, diis_space=5, out=None)
+    assert abs(r.en_cor - ref.en_cor) < 1e-9
+    eng.close()
+
+
+def test_fp32_graph_replays_match_eager(monkeypatch):
+    _say("case fp32 graph replay")
+    c = Case(16, 4, 32, 17, occ_b=3)
+    vec = _random_state(c, "CCSD", 9)
+    engines = []
+    for graph in ("1", "0"):
+        monkeypatch.setenv("CCQ_HIP_GRAPH", graph)
+        eng = _engine(c, "CCSD", shift=0.03)
+        eng.set_t(vec)
+        eng.set_precision(1)
+        engines.append(eng)
+    for rep in range(3):
+        out = [(eng.update(), eng.energy(), eng.get_t())[1:] for eng in engines]
+        assert out[0][0] == out[1][0]
+        np.testing.assert_array_equal(out[0][1], out[1][1])
+    # switching back to FP64 uses the FP64 graph again
+    for eng in engines:
+        eng.set_precision(0)
+    out = [(eng.update(), eng.get_t())[1] for eng in engines]
+    np.testing.assert_array_equal(out[0], out[1])
+    for eng in engines:
+        eng.close()
+
+
+def test_precision_mode_is_checked():
+    _say("case precision mode error")
+    c = Case(8, 3, 16, 1)
+    eng = _engine(c, "CCSD")
+    with pytest.raises(E.EngineError, match="precision"):
+        eng.set_precision(2)
+    eng.close()
