@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libccq_hip.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("engine.hip", "kernels.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("engine.hip", "kernels.hip", "guess.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

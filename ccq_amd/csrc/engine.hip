@@ -204,6 +204,7 @@ struct ccq_hip_ctx {
 
   ~ccq_hip_ctx() {
     for (auto& kv : graphs) drop_graph(kv.second);
+    if (guess_params) (void)hipFree(guess_params);
     if (d3tab) (void)hipFree(d3tab);
     if (tables32) (void)hipFree(tables32);
     if (pk_tab) (void)hipFree(pk_tab);
@@ -1362,6 +1363,91 @@ struct ccq_hip_ctx {
         for (int x = 0; x < 4; ++x) e23[x] += host_partial[((size_t)cs * kPartialBlocks + b) * 4 + x];
   }
 
+  // -------------------------------------------------------------------------
+  // starting guess for T2 (MP2 or the learned MLP correction of it)
+
+  float* guess_params = nullptr;
+
+  std::vector<ccqk::GuessBlock> guess_blocks() {
+    auto mk = [&](const char* blk, const double* V, int64_t o1, int64_t o2, int64_t v1, int64_t v2,
+                  const double* f1, const double* f2) {
+      ccqk::GuessBlock B{};
+      B.V = V;
+      B.nc = nc;
+      B.o1 = o1;
+      B.o2 = o2;
+      B.v1 = v1;
+      B.v2 = v2;
+      B.va0 = (f1 == fdiag_a.p) ? oa : ob;
+      B.vb0 = (f2 == fdiag_a.p) ? oa : ob;
+      B.f1 = f1;
+      B.f2 = f2;
+      B.t = tvec.p + blk_off.at(blk);
+      // 1 / rms of the MP2 amplitudes of the block
+      const int64_t total = o1 * o2 * v1 * v2;
+      HIPCHK(ccqk::launch_guess_msq(stream, B, partial.p, kPartialBlocks));
+      const double s2 = reduce_partials(kPartialBlocks);
+      B.inv_sm = (total > 0 && s2 > 0.0) ? 1.0 / std::sqrt(s2 / (double)total) : 0.0;
+      return B;
+    };
+    return {mk("t2aa", Vaa.p, oa, oa, va, va, fdiag_a.p, fdiag_a.p),
+            mk("t2ab", Vab.p, oa, ob, va, vb, fdiag_a.p, fdiag_b.p),
+            mk("t2bb", vbb_ptr, ob, ob, vb, vb, fdiag_b.p, fdiag_b.p)};
+  }
+
+  // mode 0: T = 0; 1: T2 = MP2 amplitudes; 2: T2 = MP2 (1 + MLP); T1, T3, T4 = 0
+  void guess(int64_t mode, const float* params, int64_t nparams) {
+    if (!ints_ready) throw Fail("integrals not uploaded");
+    if (mode < 0 || mode > 2) throw Fail("guess mode must be 0 (zero), 1 (MP2) or 2 (learned)");
+    if (mode == 2 && (!params || nparams != ccqk::kGuessParams))
+      throw Fail("learned guess needs " + std::to_string(ccqk::kGuessParams) + " parameters");
+    HIPCHK(ccqk::launch_fill(stream, tvec.p, dev_t, 0.0));
+    if (mode == 0) {
+      HIPCHK(hipStreamSynchronize(stream));
+      return;
+    }
+    if (mode == 2) {
+      if (!guess_params) HIPCHK(hipMalloc(&guess_params, sizeof(float) * ccqk::kGuessParams));
+      HIPCHK(hipMemcpyAsync(guess_params, params, sizeof(float) * ccqk::kGuessParams,
+                            hipMemcpyHostToDevice, stream));
+    }
+    for (auto& B : guess_blocks())
+      HIPCHK(ccqk::launch_guess_apply(stream, B, mode == 2 ? guess_params : nullptr));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  int64_t guess_data_size() const { return oa * oa * va * va + oa * ob * va * vb + ob * ob * vb * vb; }
+
+  // features / targets of every T2 tuple at the current amplitudes
+  void guess_data(float* X, float* M, float* Y, int64_t n) {
+    if (!ints_ready) throw Fail("integrals not uploaded");
+    if (n != guess_data_size()) throw Fail("guess data size mismatch");
+    float *dX = nullptr, *dM = nullptr, *dY = nullptr;
+    auto release = [&] {
+      if (dX) (void)hipFree(dX);
+      if (dM) (void)hipFree(dM);
+      if (dY) (void)hipFree(dY);
+    };
+    try {
+      HIPCHK(hipMalloc(&dX, sizeof(float) * std::max<int64_t>(n, 1) * ccqk::kGuessFeatures));
+      HIPCHK(hipMalloc(&dM, sizeof(float) * std::max<int64_t>(n, 1)));
+      HIPCHK(hipMalloc(&dY, sizeof(float) * std::max<int64_t>(n, 1)));
+      int64_t off = 0;
+      for (auto& B : guess_blocks()) {
+        HIPCHK(ccqk::launch_guess_data(stream, B, dX + off * ccqk::kGuessFeatures, dM + off, dY + off));
+        off += B.o1 * B.o2 * B.v1 * B.v2;
+      }
+      HIPCHK(hipStreamSynchronize(stream));
+      HIPCHK(hipMemcpy(X, dX, sizeof(float) * n * ccqk::kGuessFeatures, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(M, dM, sizeof(float) * n, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(Y, dY, sizeof(float) * n, hipMemcpyDeviceToHost));
+    } catch (...) {
+      release();
+      throw;
+    }
+    release();
+  }
+
   // the DIIS vector the ring works on: t_vec, or l_vec in left mode
   int64_t diis_len() const { return left_mode ? l_size : vec_size; }
 
@@ -1748,6 +1834,16 @@ int ccq_hip_get_l(ccq_hip_ctx* ctx, double* l_vec, int64_t l_size) {
 
 int ccq_hip_cct3(ccq_hip_ctx* ctx, double* e23) {
   return guarded(ctx, [&] { ctx->cct3(e23); });
+}
+
+int ccq_hip_guess(ccq_hip_ctx* ctx, int64_t mode, const float* params, int64_t nparams) {
+  return guarded(ctx, [&] { ctx->guess(mode, params, nparams); });
+}
+
+int64_t ccq_hip_guess_data_size(const ccq_hip_ctx* ctx) { return ctx ? ctx->guess_data_size() : -1; }
+
+int ccq_hip_guess_data(ccq_hip_ctx* ctx, float* X, float* M, float* Y, int64_t n) {
+  return guarded(ctx, [&] { ctx->guess_data(X, M, Y, n); });
 }
 
 int ccq_hip_set_precision(ccq_hip_ctx* ctx, int64_t mode) {

@@ -141,6 +141,33 @@ hipError_t launch_cct3(hipStream_t s, const Cct3Params& P, const double* L3, con
                        const int* perms, const double* signs, double* partial, int blocks);
 // dst[i] = src[i * (ld + 1)], i < n (diagonal of a square block with row stride ld)
 hipError_t launch_diag(hipStream_t s, const double* src, int64_t n, int64_t ld, double* dst);
+// ---- learned starting guess (guess.hip) ----
+// One T2 spin block t[i][j][a][b] (extents o1 o2 v1 v2) and the integrals it
+// is guessed from: V[p][q][r][s] full correlated range, C order, occupied
+// first per spin; the virtual a (b) sits at full index va0 + a (vb0 + b).
+// f1 / f2: [occ | vir] Fock diagonals of the spins of (i, a) / (j, b).
+struct GuessBlock {
+  const double* V;
+  int64_t nc;
+  int64_t o1, o2, v1, v2;
+  int64_t va0, vb0;
+  const double* f1;
+  const double* f2;
+  double inv_sm;        // 1 / rms of the MP2 amplitudes of the block
+  double* t;            // the block inside t_vec
+};
+constexpr int kGuessFeatures = 5;
+constexpr int kGuessHidden = 64;
+// packed float parameters: mu[5] sd[5] W1[5][64] b1[64] W2[64][64] b2[64] W3[64] b3
+constexpr int kGuessParams = 2 * kGuessFeatures + kGuessFeatures * kGuessHidden + kGuessHidden +
+                             kGuessHidden * kGuessHidden + kGuessHidden + kGuessHidden + 1;
+// per-block partial sums of m^2 (m = V / D, the MP2 amplitude)
+hipError_t launch_guess_msq(hipStream_t s, const GuessBlock& B, double* partial, int blocks);
+// t = m (params == nullptr) or t = m * (1 + r(features)) with the BF16 MLP
+hipError_t launch_guess_apply(hipStream_t s, const GuessBlock& B, const float* params);
+// training data of the block at its current (converged) t: features X[n][5],
+// M = m / sm, Y = (t - m) / sm, for every tuple in C order
+hipError_t launch_guess_data(hipStream_t s, const GuessBlock& B, float* X, float* M, float* Y);
 hipError_t launch_energy_pairs(hipStream_t s, const EnergyParams& P, double* partial, int blocks);
 hipError_t launch_dot_strided(hipStream_t s, int64_t rows, int64_t cols, const double* a,
                               int64_t lda, const double* b, double* partial, int blocks);

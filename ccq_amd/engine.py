@@ -64,6 +64,18 @@ EXPORTS = {
     "ccq_hip_get_l": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_cct3": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "ccq_hip_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_guess": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_guess_data_size": (ctypes.c_int64, [ctypes.c_void_p]),
+    "ccq_hip_guess_data": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 3 + [ctypes.c_int64]),
+    "ccq_hip_mlp_nparams": (ctypes.c_int64, []),
+    "ccq_hip_mlp_train": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_int64, ctypes.c_void_p,
+                                         ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_double, ctypes.c_uint64,
+                                         ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_mlp_grad": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_int64, ctypes.c_void_p,
+                                        ctypes.c_int64, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_mlp_last_error": (ctypes.c_char_p, []),
     "ccq_hip_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_profile_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "ccq_hip_debug_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -161,6 +173,22 @@ class Engine:
 
     def diis_extrapolate(self):
         self._check(self.lib.ccq_hip_diis_extrapolate(self.h), "diis_extrapolate")
+
+    def guess(self, mode: str, params=None):
+        """Starting amplitudes: "zero", "mp2" or "learned" (params: packed float32)."""
+        m = {"zero": 0, "mp2": 1, "learned": 2}[mode]
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.ccq_hip_guess(self.h, m, None if p is None else _ptr(p),
+                                           0 if p is None else p.size), "guess")
+
+    def guess_data(self):
+        """(X[n, 5], M[n], Y[n]) of every T2 tuple at the current amplitudes."""
+        n = int(self.lib.ccq_hip_guess_data_size(self.h))
+        X = np.empty((n, 5), dtype=np.float32)
+        M = np.empty(n, dtype=np.float32)
+        Y = np.empty(n, dtype=np.float32)
+        self._check(self.lib.ccq_hip_guess_data(self.h, _ptr(X), _ptr(M), _ptr(Y), n), "guess_data")
+        return X, M, Y
 
     def set_precision(self, mode: int):
         """0: FP64 MFMA contractions; 1: FP32 MFMA for the tiled contractions."""

@@ -135,6 +135,30 @@ int ccq_hip_get_l(ccq_hip_ctx* ctx, double* l_vec, int64_t l_size);
  * e23[0..3] = cc%mm_en_cor_a .. _d (CC(t;3) energy = E_cor + e23[3]). */
 int ccq_hip_cct3(ccq_hip_ctx* ctx, double* e23);
 
+/* ---- starting guess (ccq starts from T = 0, src/calc_driver.f90:210-211) ----
+ * mode 0: T = 0; 1: T2 = MP2 amplitudes <ij||ab> / D_ijab; 2: the learned
+ * guess T2 = MP2 (1 + r), r from a 5-64-64-1 BF16 MLP over per-amplitude
+ * features (|MP2| / rms, f_ii + f_jj, f_aa + f_bb, <ij||ij>, <ab||ab>).
+ * T1, T3, T4 start at zero.  params: ccq_hip_mlp_nparams() floats
+ * (mu[5] sd[5] W1[5][64] b1[64] W2[64][64] b2[64] W3[64] b3). */
+int ccq_hip_guess(ccq_hip_ctx* ctx, int64_t mode, const float* params, int64_t nparams);
+/* Training data from the current (converged) amplitudes: for every T2 tuple
+ * of the T2A, T2B, T2C blocks in order, X[n][5] features, M = MP2 / rms and
+ * Y = (t - MP2) / rms. */
+int64_t ccq_hip_guess_data_size(const ccq_hip_ctx* ctx);
+int ccq_hip_guess_data(ccq_hip_ctx* ctx, float* X, float* M, float* Y, int64_t n);
+/* GPU training of the guess network (no context; device of the calling
+ * thread): `steps` Adam steps on batches of `batch` samples drawn by a hash
+ * of (seed, step, slot); loss = mean (M r - Y)^2.  params in / out. */
+int64_t ccq_hip_mlp_nparams(void);
+int ccq_hip_mlp_train(const float* X, const float* M, const float* Y, int64_t n, float* params,
+                      int64_t nparams, int64_t steps, int64_t batch, double lr, uint64_t seed,
+                      double* final_loss);
+/* Loss and gradient over all n samples (the training kernels, one step, no update). */
+int ccq_hip_mlp_grad(const float* X, const float* M, const float* Y, int64_t n,
+                     const float* params, int64_t nparams, float* grad, double* loss);
+const char* ccq_hip_mlp_last_error(void);
+
 /* Arithmetic of the bulk contractions of later ccq_hip_update calls: 0 FP64
  * MFMA (default, ccq's precision); 1 FP32 MFMA for the LDS-tiled
  * contractions (operands rounded to FP32 on the way into LDS, FP32
