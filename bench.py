@@ -12,13 +12,15 @@ Protocol
   3. plain leg: reset T = 0 and time exactly K FP64 iterations (update,
      energy, DIIS ring, residuum, DIIS every 6th), synchronising the stream
      around the timed region and after every iteration;
-  4. mixed leg: the same K iterations with FP32-MFMA contractions until
-     |dE| < 1e-6, FP64 after that (the FP64 refinement);
-  5. value = mixed leg's wall time until the first iteration with
-     |E - E_conv| < 1e-7; the plain leg's time, the speedup, and the mixed
-     leg's final energy and max-abs T2 error against the FP64 solution go
-     into config.  (The learned starting guess is not built; both legs start
-     from T = 0.)
+  4. learned leg: the learned T2 guess (ccq_amd/guess_weights.npz; the MP2
+     guess if absent), computed inside the timed region, then K FP64
+     iterations;
+  5. mixed+learned leg: the learned guess, then K iterations with FP32-MFMA
+     contractions until |dE| < 1e-6 and FP64 after that (the refinement);
+  6. value = the mixed+learned leg's wall time until the first iteration
+     with |E - E_conv| < 1e-7; every leg's time, iteration count, final
+     energy and max-abs T error against the FP64 solution, and the speedup
+     over the plain leg go into config.
   Roofline: the dominant GEMM class (the contraction shape with the largest
   share of the sweep) timed with HIP events on the engine's stream during one
   extra profiled iteration; algorithmic FLOPs 2*M*N*K per launch.
@@ -44,6 +46,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from ccq_amd import engine as E  # noqa: E402
+from ccq_amd import guess  # noqa: E402
 from ccq_amd import synth  # noqa: E402
 from ccq_amd.inputs import build_integrals  # noqa: E402
 
@@ -193,16 +196,23 @@ def main():
     run_iterations(eng, args.warmup, diis)
     eng.set_t(zero)
     run_iterations(eng, args.warmup, diis, switch_de=0.0)
+    params, guess_kind = None, "mp2"
+    if os.path.exists(guess.WEIGHTS):
+        params, guess_kind = guess.load_weights(), "learned"
+    eng.guess(guess_kind, params)
     eng.set_t(zero)
     e_conv, it_conv = converge(eng, diis)
     t_conv = eng.get_t()
     log(f"E_conv = {e_conv:.12f} after {it_conv} iterations")
     legs = {}
-    for leg, switch in (("plain", None), ("mixed", SWITCH_DE)):
+    for leg, switch, start_guess in (("plain", None, None), ("learned", None, guess_kind),
+                                      ("mixed+learned", SWITCH_DE, guess_kind)):
         eng.set_t(zero)
         energies, stamps = [], []
         eng.sync()
         start = time.perf_counter()
+        if start_guess:
+            eng.guess(start_guess, params)
         run_iterations(eng, args.steps, diis, energies, stamps, switch_de=switch)
         eng.sync()
         total = time.perf_counter() - start
@@ -215,9 +225,9 @@ def main():
         log(f"{leg}: {args.steps} iterations {total * 1e3 / args.steps:.3f} ms/iter; "
             f"|E-E_conv|<1e-7 at iteration {legs[leg]['n_needed']}: {legs[leg]['value']}; "
             f"final |dE| {legs[leg]['e_err']:.2e}, max|dT| {legs[leg]['t2_err']:.2e}")
-    value = legs["mixed"]["value"]
+    value = legs["mixed+learned"]["value"]
     n_needed = legs["plain"]["n_needed"]
-    total = legs["mixed"]["total"]
+    total = legs["mixed+learned"]["total"]
     # roofline of the dominant kernel: one profiled sweep (HIP events on the stream)
     eng.profile(True)
     eng.update()
@@ -261,15 +271,12 @@ def main():
                                f"{synth.WORKLOADS[args.workload]['seed']}, diis 5, "
                                f"{'rhf' if args.rhf else 'uhf'} sweep",
                    "e_conv": e_conv,
-                   "mode": "mixed: FP32-MFMA tiled contractions until |dE|<1e-6, then FP64; "
-                           "starting guess T=0 (learned guess not built)",
-                   "plain_fp64": {"time_to_1e-7_s": legs["plain"]["value"],
-                                  "iterations_to_1e-7": legs["plain"]["n_needed"],
-                                  "ms_per_step": legs["plain"]["total"] * 1e3 / args.steps},
-                   "mixed": {"time_to_1e-7_s": legs["mixed"]["value"],
-                             "iterations_to_1e-7": legs["mixed"]["n_needed"],
-                             "final_energy_error": legs["mixed"]["e_err"],
-                             "final_max_abs_t_error": legs["mixed"]["t2_err"]},
+                   "mode": f"mixed+learned: {guess_kind} T2 guess, FP32-MFMA tiled contractions "
+                           f"until |dE|<{SWITCH_DE:g}, then FP64",
+                   "legs": {k: {"time_to_1e-7_s": v["value"], "iterations_to_1e-7": v["n_needed"],
+                                "ms_per_step": v["total"] * 1e3 / args.steps,
+                                "final_energy_error": v["e_err"],
+                                "final_max_abs_t_error": v["t2_err"]} for k, v in legs.items()},
                    "speedup_vs_plain": (legs["plain"]["value"] / value
                                         if value and legs["plain"]["value"] else None)},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
