@@ -914,7 +914,12 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
   // Cost model in cycles (2.4 GHz).  A 16x16x4 FP64 MFMA occupies a SIMD for
   // 64 cycles (78.6 TF/s over 256 CUs x 4 SIMDs).
   const double cus = 256.0;
-  const double hbm = 8.0e12 / 2.4e9;   // bytes per cycle
+  // streaming rates measured for the two kernels on skinny products
+  // (bytes per cycle): the LDS-tiled loaders read k-contiguous rows with
+  // coalesced 256-byte runs (~4 TB/s), the register-direct fragment loads
+  // reach ~2.5 TB/s
+  const double hbm_tile = 4.0e12 / 2.4e9;
+  const double hbm_reg = 2.5e12 / 2.4e9;
   double best = 1e300;
   auto take = [&](double cyc, int kind, int trans, int bm, int bn, int bk, const RegShape* r,
                   int64_t split, int64_t kchunk) {
@@ -934,15 +939,24 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
 
   // LDS-tiled kernel.  A WG of 4 waves keeps each SIMD busy for
   // (BM/32)(BN/32)*BK/4 MFMAs per K slab; resident WGs per CU (VGPR/LDS):
-  // 2 for 128x128, 3 for 128x64 / 64x128, 4 for 64x64.  A CU holding a single
-  // WG cannot hide the global-load latency of a slab (~1500 cycles); every
-  // slab costs two barriers and the LDS round trip (~300 cycles).
+  // 2 for 128x128, 3 for 128x64 / 64x128, 4 for 64x64, 6 for 64x32 / 32x64,
+  // 8 for 32x32.  A CU holding a single WG cannot hide the global-load
+  // latency of a slab (~1500 cycles); every slab costs two barriers and the
+  // LDS round trip (~300 cycles).  The small tiles pay off for products with
+  // a short side of a few MFMA tiles or a short K (more WGs, no split-K).
   struct Shape { int bm, bn, occ; };
   // resident WGs per CU (128x128 uses more than 256 VGPRs: one)
-  const Shape shapes[4] = {{128, 128, 1}, {128, 64, 3}, {64, 128, 3}, {64, 64, 4}};
+  const Shape shapes[7] = {{128, 128, 1}, {128, 64, 3}, {64, 128, 3}, {64, 64, 4},
+                           {64, 32, 6},   {32, 64, 6},  {32, 32, 8}};
   for (int bk : {16, 32}) {
     const int64_t ksteps = cdiv(g.K, bk);
     for (const Shape& sh : shapes) {
+      // small tiles only for a short side (measured: no gain, or a loss, on
+      // square-ish products such as the 900^3 ring)
+      if (sh.bm * sh.bn < 64 * 64 && std::min(g.M, g.N) > 128) continue;
+      // no tiles for dot-like products with a tiny side and a long K (the
+      // register-direct kernel with split-K measured faster)
+      if (std::min(g.M, g.N) < 16 && g.K >= 4096) continue;
       const int64_t tiles = cdiv(g.M, sh.bm) * cdiv(g.N, sh.bn);
       const double slab = (sh.bm / 32.0) * (sh.bn / 32.0) * (bk / 4.0) * 64.0;
       const int64_t maxsplit = std::max<int64_t>(1, std::min<int64_t>(ksteps / 2, 512));
@@ -953,6 +967,12 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
         const double resident = std::min<double>(per_cu, sh.occ);
         double cyc = per_cu * (steps + 2) * (slab + 300.0);
         if (resident < 2) cyc += per_cu * steps * 1500.0;
+        if (std::min(g.M, g.N) <= 128) {
+          // skinny: at least one streaming pass over the operands and C
+          const double bytes = 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N);
+          // + the epilogue's table-addressed stores (tile elements per thread)
+          cyc = std::max(cyc, bytes / hbm_tile + per_cu * (sh.bm * sh.bn / 256.0) * 8.0);
+        }
         cyc += splitk_cycles(g.M, g.N, split);
         take(cyc, 0, 0, sh.bm, sh.bn, bk, nullptr, split, steps * bk);
       }
@@ -981,8 +1001,10 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
         const double wps = (double)tiles * split * 4.0 / (cus * 4.0);   // waves per SIMD
         const double rounds = std::ceil(wps / occ);
         const double busy = std::min(wps, occ);
-        double cyc = rounds * (steps + 1) * std::max(3000.0, mfma * busy);
-        cyc = std::max(cyc, bytes / hbm);
+        const double lat = rounds * (steps + 1) * std::max(3000.0, mfma * busy);
+        // streaming floor; the latency model still orders the shapes when
+        // the floor dominates
+        double cyc = std::max(lat, bytes / hbm_reg) + 0.05 * lat;
         cyc += splitk_cycles(g.M, g.N, split);
         take(cyc, 1, trans, 64, 64, 16, &r, split, cdiv(k16, split) * 16);
       }
@@ -1060,16 +1082,29 @@ static hipError_t launch_kernels(hipStream_t s, const GemmArgs& g) {
     return hipGetLastError();
   }
   dim3 grid((unsigned)cdiv(g.M, g.bm), (unsigned)cdiv(g.N, g.bn), (unsigned)g.nsplit);
+  const int key = g.bm * 1000 + g.bn;
   if (g.bk == 32) {
-    if (g.bm == 128 && g.bn == 128) launch_tile<128, 128, 32>(s, grid, g);
-    else if (g.bm == 128) launch_tile<128, 64, 32>(s, grid, g);
-    else if (g.bn == 128) launch_tile<64, 128, 32>(s, grid, g);
-    else launch_tile<64, 64, 32>(s, grid, g);
+    switch (key) {
+      case 128128: launch_tile<128, 128, 32>(s, grid, g); break;
+      case 128064: launch_tile<128, 64, 32>(s, grid, g); break;
+      case 64128: launch_tile<64, 128, 32>(s, grid, g); break;
+      case 64064: launch_tile<64, 64, 32>(s, grid, g); break;
+      case 64032: launch_tile<64, 32, 32>(s, grid, g); break;
+      case 32064: launch_tile<32, 64, 32>(s, grid, g); break;
+      case 32032: launch_tile<32, 32, 32>(s, grid, g); break;
+      default: return hipErrorInvalidValue;
+    }
   } else {
-    if (g.bm == 128 && g.bn == 128) launch_tile<128, 128, 16>(s, grid, g);
-    else if (g.bm == 128) launch_tile<128, 64, 16>(s, grid, g);
-    else if (g.bn == 128) launch_tile<64, 128, 16>(s, grid, g);
-    else launch_tile<64, 64, 16>(s, grid, g);
+    switch (key) {
+      case 128128: launch_tile<128, 128, 16>(s, grid, g); break;
+      case 128064: launch_tile<128, 64, 16>(s, grid, g); break;
+      case 64128: launch_tile<64, 128, 16>(s, grid, g); break;
+      case 64064: launch_tile<64, 64, 16>(s, grid, g); break;
+      case 64032: launch_tile<64, 32, 16>(s, grid, g); break;
+      case 32064: launch_tile<32, 64, 16>(s, grid, g); break;
+      case 32032: launch_tile<32, 32, 16>(s, grid, g); break;
+      default: return hipErrorInvalidValue;
+    }
   }
   return hipGetLastError();
 }
