@@ -168,7 +168,7 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ccsd-n100", choices=sorted(
-        k for k, v in synth.WORKLOADS.items() if v["method"] == "CCSD"))
+        k for k, v in synth.WORKLOADS.items() if v["method"] in ("CCSD", "CCSDT", "CCSDTQ")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rhf", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -186,7 +186,8 @@ def main():
     s, ints = make_workload(args.workload)
     o, n = s.nocc, s.norb
     log(f"workload built in {time.perf_counter() - t0:.1f}s: n={n} o={o} v={n - o}")
-    eng = E.Engine(0, o, o, n, "CCSD", rhf=args.rhf, diis_space=diis)
+    method = synth.WORKLOADS[args.workload]["method"]
+    eng = E.Engine(0, o, o, n, method, rhf=args.rhf, diis_space=diis)
     t0 = time.perf_counter()
     eng.set_ints(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb)
     log(f"upload {time.perf_counter() - t0:.1f}s")
@@ -249,7 +250,7 @@ def main():
     for k, v in top:
         log(f"  {k:36s} calls={v['calls']:4d} ms={v['ms']:.3f} avg_us={v['ms'] * 1e3 / v['calls']:.1f}")
     cpu = None
-    if not args.no_cpu and n_needed is not None:
+    if not args.no_cpu and n_needed is not None and method == "CCSD":
         log("cpu baseline (oracle) ...")
         cpu = cpu_baseline(s, ints, n_needed)
         log(f"cpu baseline {cpu['value']:.1f} s ({cpu['sample']})")
@@ -266,7 +267,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64/f32 mixed",
         "data": "synthetic (seeded integrals, ccq onebody/twobody construction)",
-        "config": {"workload": f"{args.workload}: CCSD, n={n} o={o} v={n - o}, "
+        "config": {"workload": f"{args.workload}: {method}, n={n} o={o} v={n - o}, "
                                f"L={synth.WORKLOADS[args.workload]['nL']}, seed "
                                f"{synth.WORKLOADS[args.workload]['seed']}, diis 5, "
                                f"{'rhf' if args.rhf else 'uhf'} sweep",

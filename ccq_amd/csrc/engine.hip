@@ -1390,9 +1390,11 @@ struct ccq_hip_ctx {
       B.inv_sm = (total > 0 && s2 > 0.0) ? 1.0 / std::sqrt(s2 / (double)total) : 0.0;
       return B;
     };
-    return {mk("t2aa", Vaa.p, oa, oa, va, va, fdiag_a.p, fdiag_a.p),
-            mk("t2ab", Vab.p, oa, ob, va, vb, fdiag_a.p, fdiag_b.p),
-            mk("t2bb", vbb_ptr, ob, ob, vb, vb, fdiag_b.p, fdiag_b.p)};
+    std::vector<ccqk::GuessBlock> out = {mk("t2aa", Vaa.p, oa, oa, va, va, fdiag_a.p, fdiag_a.p),
+                                         mk("t2ab", Vab.p, oa, ob, va, vb, fdiag_a.p, fdiag_b.p),
+                                         mk("t2bb", vbb_ptr, ob, ob, vb, vb, fdiag_b.p, fdiag_b.p)};
+    out[0].anti = out[2].anti = 1;
+    return out;
   }
 
   // mode 0: T = 0; 1: T2 = MP2 amplitudes; 2: T2 = MP2 (1 + MLP); T1, T3, T4 = 0

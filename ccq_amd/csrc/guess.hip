@@ -54,13 +54,7 @@ struct Tuple {
   float x[F];
 };
 
-__device__ inline Tuple guess_tuple(const GuessBlock& B, int64_t e, int64_t& off) {
-  int64_t r = e;
-  const int64_t b = r % B.v2; r /= B.v2;
-  const int64_t a = r % B.v1; r /= B.v1;
-  const int64_t j = r % B.o2; r /= B.o2;
-  const int64_t i = r;
-  off = e;
+__device__ inline Tuple tuple_at(const GuessBlock& B, int64_t i, int64_t j, int64_t a, int64_t b) {
   const int64_t nc = B.nc, nc2 = nc * nc, nc3 = nc2 * nc;
   const int64_t A = B.va0 + a, Bv = B.vb0 + b;
   const double V = B.V[i * nc3 + j * nc2 + A * nc + Bv];
@@ -74,6 +68,27 @@ __device__ inline Tuple guess_tuple(const GuessBlock& B, int64_t e, int64_t& off
   t.x[3] = (float)B.V[i * nc3 + j * nc2 + i * nc + j];
   t.x[4] = (float)B.V[A * nc3 + Bv * nc2 + A * nc + Bv];
   return t;
+}
+
+__device__ inline Tuple guess_tuple(const GuessBlock& B, int64_t e, int64_t& off) {
+  int64_t r = e;
+  const int64_t b = r % B.v2; r /= B.v2;
+  const int64_t a = r % B.v1; r /= B.v1;
+  const int64_t j = r % B.o2; r /= B.o2;
+  const int64_t i = r;
+  off = e;
+  return tuple_at(B, i, j, a, b);
+}
+
+// p-th pair (x < y) of n in row order (0,1), (0,2), ..., (1,2), ...
+__device__ inline void pair_of(int64_t p, int64_t n, int64_t& x, int64_t& y) {
+  const int64_t total = n * (n - 1) / 2;
+  const int64_t q = total - 1 - p;   // pairs counted from the end
+  int64_t k = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) * 0.5);
+  while ((k + 1) * (k + 2) / 2 <= q) ++k;
+  while (k * (k + 1) / 2 > q) --k;
+  x = n - 2 - k;
+  y = n - 1 - (q - k * (k + 1) / 2);
 }
 
 __global__ __launch_bounds__(256) void guess_msq(GuessBlock B, double* __restrict__ partial) {
@@ -138,6 +153,27 @@ __device__ inline void load_weights(float* w, const float* __restrict__ params) 
 __global__ __launch_bounds__(256) void guess_apply(GuessBlock B, const float* __restrict__ params) {
   __shared__ float w[kGuessParams];
   if (params) load_weights(w, params);
+  if (B.anti) {
+    // t[ijab] = -t[jiab] = -t[ijba] = t[jiba]; the network sees features
+    // that are even under both swaps, so one evaluation serves all four
+    const int64_t po = B.o1 * (B.o1 - 1) / 2, pv = B.v1 * (B.v1 - 1) / 2;
+    const int64_t total = po * pv;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+      int64_t i, j, a, b;
+      pair_of(e / pv, B.o1, i, j);
+      pair_of(e % pv, B.v1, a, b);
+      const Tuple t = tuple_at(B, i, j, a, b);
+      double v = t.m;
+      if (params) v = t.m * (1.0 + (double)mlp_forward(w, t.x, nullptr));
+      const int64_t o = B.o2, vv = B.v2;
+      B.t[((i * o + j) * B.v1 + a) * vv + b] = v;
+      B.t[((j * o + i) * B.v1 + a) * vv + b] = -v;
+      B.t[((i * o + j) * B.v1 + b) * vv + a] = -v;
+      B.t[((j * o + i) * B.v1 + b) * vv + a] = v;
+    }
+    return;
+  }
   const int64_t total = B.o1 * B.o2 * B.v1 * B.v2;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
@@ -174,7 +210,10 @@ hipError_t launch_guess_msq(hipStream_t s, const GuessBlock& B, double* partial,
 }
 
 hipError_t launch_guess_apply(hipStream_t s, const GuessBlock& B, const float* params) {
-  const int64_t total = B.o1 * B.o2 * B.v1 * B.v2;
+  if (B.anti && (B.o1 != B.o2 || B.v1 != B.v2 || B.va0 != B.vb0 || B.f1 != B.f2))
+    return hipErrorInvalidValue;
+  const int64_t total = B.anti ? (B.o1 * (B.o1 - 1) / 2) * (B.v1 * (B.v1 - 1) / 2)
+                               : B.o1 * B.o2 * B.v1 * B.v2;
   if (total <= 0) return hipSuccess;
   hipLaunchKernelGGL(guess_apply, dim3(grid_of(total)), dim3(256), 0, s, B, params);
   return hipGetLastError();

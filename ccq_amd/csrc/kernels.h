@@ -155,6 +155,7 @@ struct GuessBlock {
   const double* f2;
   double inv_sm;        // 1 / rms of the MP2 amplitudes of the block
   double* t;            // the block inside t_vec
+  int anti = 0;         // same-spin block: evaluate i<j, a<b and write the images
 };
 constexpr int kGuessFeatures = 5;
 constexpr int kGuessHidden = 64;
