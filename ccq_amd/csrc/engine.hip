@@ -744,7 +744,8 @@ struct ccq_hip_ctx {
       t = "r" + std::to_string(g.rm) + std::to_string(g.rn) + "w" + std::to_string(g.wk) +
           (g.trans ? "t" : "");
     else
-      t = std::to_string(g.bm) + "x" + std::to_string(g.bn) + "k" + std::to_string(g.bk);
+      t = std::to_string(g.bm) + "x" + std::to_string(g.bn) + "k" + std::to_string(g.bk) +
+          (g.ksplit == 2 ? "d" : "");
     return t + "s" + std::to_string(g.nsplit);
   }
 

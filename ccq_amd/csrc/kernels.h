@@ -38,6 +38,9 @@ struct GemmArgs {
   // arithmetic of the LDS-tiled kernel: 0 FP64 MFMA; 1 FP32 MFMA (operands
   // rounded to FP32 when staged into LDS, FP32 accumulation, FP64 epilogue)
   int prec = 0;
+  // LDS-tiled kernel: 2 = two 4-wave groups share a tile and split its K
+  // slabs (tiles up to 64x64), 1 = one group
+  int ksplit = 1;
   // split-K workspace of at least gemm_ws_need doubles
   double* ws = nullptr;
 };

@@ -185,18 +185,26 @@ struct Mma<float> {
 // on v_mfma_f64_16x16x4f64; T = float rounds the operands to FP32 and
 // accumulates in FP32 on v_mfma_f32_16x16x4f32 (the mixed-precision form).
 // blockIdx.z owns K range [z*kchunk, (z+1)*kchunk) (split-K, completed by
-// gemm_splitk_reduce).
-template <typename T, int BM, int BN, int BK, bool AKF, bool BKF>
-__global__ __launch_bounds__(256, (sizeof(T) == 8 && BM * BN >= 128 * 128) ? 1 : 2) void gemm_tab(
-    GemmArgs g) {
+// gemm_splitk_reduce).  KS = 2 doubles the workgroup: two groups of 4 waves
+// take alternate K slabs of the same tile (each with its own LDS slab) and
+// combine their accumulators through LDS at the end in a fixed order -- the
+// K split of a global split-K without its partial products in HBM and its
+// reduction launch.
+template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+__global__ __launch_bounds__(256 * KS, (KS == 1 && !(sizeof(T) == 8 && BM * BN >= 128 * 128)) ? 2 : 1)
+void gemm_tab(GemmArgs g) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   using TA = LdsTile<T, BM, AKF, BK>;
   using TB = LdsTile<T, BN, BKF, BK>;
   using MM = Mma<T>;
-  __shared__ T As[TA::size];
-  __shared__ T Bs[TB::size];
-  const int tid = threadIdx.x;
+  constexpr int TILE_BYTES = KS * (TA::size + TB::size) * (int)sizeof(T);
+  constexpr int RED_BYTES = (KS - 1) * 256 * TM * TN * 4 * (int)sizeof(T);
+  __shared__ __align__(16) unsigned char smem[TILE_BYTES > RED_BYTES ? TILE_BYTES : RED_BYTES];
+  const int grp = threadIdx.x >> 8;
+  T* As = reinterpret_cast<T*>(smem) + grp * (TA::size + TB::size);
+  T* Bs = As + TA::size;
+  const int tid = threadIdx.x & 255;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   int tx, ty, tz;
@@ -220,10 +228,13 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && BM * BN >= 128 * 128) ? 1 :
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = typename MM::acc_t{0, 0, 0, 0};
 
-  const int64_t nsteps = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  // slabs of this tile's K range; group grp takes slabs grp, grp + KS, ...
+  // (a slab past the end loads zeros, so both groups run the same barriers)
+  const int64_t nslabs = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  const int64_t nsteps = (nslabs + KS - 1) / KS;
   if (nsteps > 0) {
-    la.fetch(ra, g.A, g.kA, g.kA_aff, kbeg, kend);
-    lb.fetch(rb, g.B, g.kB, g.kB_aff, kbeg, kend);
+    la.fetch(ra, g.A, g.kA, g.kA_aff, kbeg + grp * BK, kend);
+    lb.fetch(rb, g.B, g.kB, g.kB_aff, kbeg + grp * BK, kend);
   }
   for (int64_t s = 0; s < nsteps; ++s) {
     __syncthreads();
@@ -231,8 +242,9 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && BM * BN >= 128 * 128) ? 1 :
     lb.stash(Bs, rb);
     __syncthreads();
     if (s + 1 < nsteps) {
-      la.fetch(ra, g.A, g.kA, g.kA_aff, kbeg + (s + 1) * BK, kend);
-      lb.fetch(rb, g.B, g.kB, g.kB_aff, kbeg + (s + 1) * BK, kend);
+      const int64_t k0 = kbeg + ((s + 1) * KS + grp) * BK;
+      la.fetch(ra, g.A, g.kA, g.kA_aff, k0, kend);
+      lb.fetch(rb, g.B, g.kB, g.kB_aff, k0, kend);
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
@@ -248,6 +260,29 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && BM * BN >= 128 * 128) ? 1 :
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = MM::step(af[i], bf[j], acc[i][j]);
     }
+  }
+
+  if (KS > 1) {
+    // group 1 hands its accumulators to group 0 (slab order: group 0's sum
+    // plus group 1's sum, the same for every launch)
+    T* red = reinterpret_cast<T*>(smem);
+    __syncthreads();
+    if (grp == 1) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) red[(((i * TN + j) * 4 + v) << 8) + tid] = acc[i][j][v];
+    }
+    __syncthreads();
+    if (grp == 1) return;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[i][j][v] += red[(((i * TN + j) * 4 + v) << 8) + tid];
   }
 
 #pragma unroll
@@ -960,12 +995,20 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
       const int64_t tiles = cdiv(g.M, sh.bm) * cdiv(g.N, sh.bn);
       const double slab = (sh.bm / 32.0) * (sh.bn / 32.0) * (bk / 4.0) * 64.0;
       const int64_t maxsplit = std::max<int64_t>(1, std::min<int64_t>(ksteps / 2, 512));
+      for (int ksp = 1; ksp <= 2; ++ksp)
       for (int64_t split = 1; split <= maxsplit; split *= 2) {
+        if (ksp == 2 && sh.bm * sh.bn > 64 * 64) break;
+        // measured inside the sweep: the doubled workgroup pays off only for
+        // matrix-vector products (a side of 1 or 2); on the 900^3 ring it
+        // beats split-K with hot operands (58 vs 61 us) but loses with the
+        // sweep's cold ones (94 vs 82 us)
+        if (ksp == 2 && std::min(g.M, g.N) > 2) break;
         if (split > 1 && split * g.M * g.N > ws_cap) break;
-        const int64_t steps = cdiv(ksteps, split);
+        // ksp = 2: a doubled workgroup whose two groups take alternate slabs
+        const int64_t steps = cdiv(ksteps, split * ksp);
         const double per_cu = std::ceil((double)tiles * split / cus);
-        const double resident = std::min<double>(per_cu, sh.occ);
-        double cyc = per_cu * (steps + 2) * (slab + 300.0);
+        const double resident = std::min<double>(per_cu, std::max(1, sh.occ / ksp)) * ksp;
+        double cyc = per_cu * (steps + 2) * (ksp * slab + 300.0);
         if (resident < 2) cyc += per_cu * steps * 1500.0;
         if (std::min(g.M, g.N) <= 128) {
           // skinny: at least one streaming pass over the operands and C
@@ -974,7 +1017,9 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
           cyc = std::max(cyc, bytes / hbm_tile + per_cu * (sh.bm * sh.bn / 256.0) * 8.0);
         }
         cyc += splitk_cycles(g.M, g.N, split);
-        take(cyc, 0, 0, sh.bm, sh.bn, bk, nullptr, split, steps * bk);
+        const double before = best;
+        take(cyc, 0, 0, sh.bm, sh.bn, bk, nullptr, split, steps * ksp * bk);
+        if (best < before) g.ksplit = ksp;
       }
     }
   }
@@ -1006,7 +1051,9 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
         // the floor dominates
         double cyc = std::max(lat, bytes / hbm_reg) + 0.05 * lat;
         cyc += splitk_cycles(g.M, g.N, split);
+        const double before = best;
         take(cyc, 1, trans, 64, 64, 16, &r, split, cdiv(k16, split) * 16);
+        if (best < before) g.ksplit = 1;
       }
     }
   }
@@ -1018,22 +1065,35 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
   g.group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, (int64_t)std::lround(std::sqrt(per_xcd))));
 }
 
-template <typename T, int BM, int BN, int KS>
-static void launch_tile_t(hipStream_t s, dim3 grid, const GemmArgs& g) {
+template <typename T, int BM, int BN, int BK, int KS>
+static void launch_tile_k(hipStream_t s, dim3 grid, const GemmArgs& g) {
+  const dim3 blk(256 * KS);
   if (g.a_kfast && g.b_kfast)
-    hipLaunchKernelGGL((gemm_tab<T, BM, BN, KS, true, true>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm_tab<T, BM, BN, BK, true, true, KS>), grid, blk, 0, s, g);
   else if (g.a_kfast)
-    hipLaunchKernelGGL((gemm_tab<T, BM, BN, KS, true, false>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm_tab<T, BM, BN, BK, true, false, KS>), grid, blk, 0, s, g);
   else if (g.b_kfast)
-    hipLaunchKernelGGL((gemm_tab<T, BM, BN, KS, false, true>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm_tab<T, BM, BN, BK, false, true, KS>), grid, blk, 0, s, g);
   else
-    hipLaunchKernelGGL((gemm_tab<T, BM, BN, KS, false, false>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((gemm_tab<T, BM, BN, BK, false, false, KS>), grid, blk, 0, s, g);
 }
 
-template <int BM, int BN, int KS>
+// the doubled workgroup exists for the tiles up to 64x64
+template <typename T, int BM, int BN, int BK>
+static void launch_tile_t(hipStream_t s, dim3 grid, const GemmArgs& g) {
+  if constexpr (BM * BN <= 64 * 64) {
+    if (g.ksplit == 2) {
+      launch_tile_k<T, BM, BN, BK, 2>(s, grid, g);
+      return;
+    }
+  }
+  launch_tile_k<T, BM, BN, BK, 1>(s, grid, g);
+}
+
+template <int BM, int BN, int BK>
 static void launch_tile(hipStream_t s, dim3 grid, const GemmArgs& g) {
-  if (g.prec == 1) launch_tile_t<float, BM, BN, KS>(s, grid, g);
-  else launch_tile_t<double, BM, BN, KS>(s, grid, g);
+  if (g.prec == 1) launch_tile_t<float, BM, BN, BK>(s, grid, g);
+  else launch_tile_t<double, BM, BN, BK>(s, grid, g);
 }
 
 template <int RM, int RN, int WK>
