@@ -95,6 +95,12 @@ struct Op {
   int64_t t_rowA = 0, t_kA = 0, t_kB = 0, t_colB = 0, t_rowC = 0, t_colC = 0;
   ccqk::GemmArgs plan{};      // launch plan (kernel, tile, split-K, affine strides)
   bool flat = false;          // add op over identical dense layouts
+  // an add op folded into the epilogue of the GEMM that next updates its
+  // output (fused: this add is not launched; fuse_a: the GEMM's addend)
+  bool fused = false;
+  int fuse_a = -1;
+  double dcoef = 0.0;
+  int64_t t_rowD = 0, t_colD = 0;
   double flops = 0, bytes = 0;
   std::string cls;           // profile class
 };
@@ -861,6 +867,7 @@ struct ccq_hip_ctx {
         }
       }
     }
+    fuse_adds(push32);
     ws.alloc(std::max<int64_t>(ws_need, 1));
     if (lsq_n) lsq.alloc(lsq_n);
     HIPCHK(hipMalloc(&tables32, sizeof(int) * std::max<size_t>(host32.size(), 1)));
@@ -873,10 +880,83 @@ struct ccq_hip_ctx {
       HIPCHK(hipMemcpy(dtab, dh.data(), sizeof(double) * dh.size(), hipMemcpyHostToDevice));
   }
 
+  // Fold an add op (C (+)= c A) into the GEMM that next updates C, when that
+  // GEMM accumulates into C, does not read C, and A is not written in
+  // between: the GEMM's epilogue adds c A (C = c A + alpha A'B' for an
+  // assigning add).  Saves the add's launch and its pass over C.
+  // CCQ_HIP_FUSE=0 keeps the adds.
+  bool fuse = true;
+
+  template <typename Push>
+  void fuse_adds(Push&& push32) {
+    if (!fuse) return;
+    for (auto& sec : sections) {
+      if (sec.kind != Section::Compute) continue;
+      auto& ops = sec.ops;
+      for (size_t i = 0; i < ops.size(); ++i) {
+        Op& ad = ops[i];
+        if (ad.gemm || ad.fused || ad.a == ad.out) continue;
+        size_t j = i + 1;
+        for (; j < ops.size(); ++j) {
+          const Op& q = ops[j];
+          if (q.out == ad.out || q.a == ad.out || (q.gemm && q.b == ad.out)) break;
+        }
+        if (j == ops.size()) continue;
+        Op& gm = ops[j];
+        if (!gm.gemm || gm.out != ad.out || gm.assign || gm.fuse_a >= 0 || gm.a == ad.out ||
+            gm.b == ad.out)
+          continue;
+        bool clobbered = false;
+        for (size_t k = i + 1; k < j; ++k)
+          if (ops[k].out == ad.a) clobbered = true;
+        if (clobbered) continue;
+        // offsets of A for the GEMM's row / column label sets of C: C position
+        // p carries add label lc[p], read from A at la.find(lc[p])
+        const Tensor& A = tensors[ad.a];
+        const Tensor& C = tensors[gm.out];
+        auto table = [&](bool rows) {
+          std::vector<int64_t> ext, str;
+          for (size_t p = 0; p < gm.lc.size(); ++p) {
+            const char c = gm.lc[p];
+            const bool in_a = gm.la.find(c) != std::string::npos;
+            if (in_a != rows) continue;
+            const auto q = ad.la.find(ad.lc[p]);
+            if (q == std::string::npos) throw Fail("add op label mismatch");
+            ext.push_back(C.dims[p]);
+            str.push_back(A.strides[q]);
+          }
+          int64_t total = 1;
+          for (auto e : ext) total *= e;
+          std::vector<int64_t> out((size_t)std::max<int64_t>(total, 1), 0);
+          std::vector<int64_t> idx(ext.size(), 0);
+          for (int64_t t = 0; t < total; ++t) {
+            int64_t o = 0;
+            for (size_t k = 0; k < ext.size(); ++k) o += idx[k] * str[k];
+            out[t] = o;
+            for (int k = (int)ext.size() - 1; k >= 0; --k) {
+              if (++idx[k] < ext[k]) break;
+              idx[k] = 0;
+            }
+          }
+          return out;
+        };
+        auto rowD = table(true), colD = table(false);
+        if ((int64_t)rowD.size() != gm.M || (int64_t)colD.size() != gm.N) continue;
+        gm.t_rowD = push32(rowD);
+        gm.t_colD = push32(colD);
+        gm.fuse_a = ad.a;
+        gm.dcoef = ad.coef;
+        gm.assign = ad.assign;
+        ad.fused = true;
+      }
+    }
+  }
+
   // -------------------------------------------------------------------------
   // execution
 
   void run_op(const Op& op) {
+    if (op.fused) return;
     const Tensor& C = tensors[op.out];
     const Tensor& A = tensors[op.a];
     const int acc = op.assign ? 0 : 1;
@@ -896,6 +976,12 @@ struct ccq_hip_ctx {
       g.accumulate = acc;
       g.ws = ws.p;
       g.prec = prec;
+      if (op.fuse_a >= 0) {
+        g.D = tensors[op.fuse_a].ptr;
+        g.rowD = tables32 + op.t_rowD;
+        g.colD = tables32 + op.t_colD;
+        g.dcoef = op.dcoef;
+      }
       HIPCHK(ccqk::launch_gemm(stream, g));
     } else {
       const bool aligned =
@@ -1055,7 +1141,8 @@ struct ccq_hip_ctx {
       if (sec.kind == Section::Cct3) {
         timed("cct3:" + sec.upd.block, 0, 0, [&] { run_cct3(sec.upd); });
       } else if (sec.kind == Section::Compute) {
-        for (auto& op : sec.ops) timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
+        for (auto& op : sec.ops)
+          if (!op.fused) timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
       } else if (sec.kind == Section::Update) {
         const Tensor& R = tensors[sec.upd.resid];
         double n = 1;
@@ -1556,6 +1643,7 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (ctx->diis < 1 || ctx->diis > ccqk::kMaxDiis) throw Fail("diis_space must be in 1..12");
     ctx->device = (int)opts->device;
     if (const char* g = std::getenv("CCQ_HIP_GRAPH")) ctx->use_graph = std::string(g) != "0";
+    if (const char* g = std::getenv("CCQ_HIP_FUSE")) ctx->fuse = std::string(g) != "0";
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->layout();

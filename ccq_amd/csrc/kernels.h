@@ -43,6 +43,12 @@ struct GemmArgs {
   int ksplit = 1;
   // split-K workspace of at least gemm_ws_need doubles
   double* ws = nullptr;
+  // optional addend folded into the epilogue: C (+)= alpha A B + dcoef D,
+  // D read at D[rowD[m] + colD[n]]
+  const double* D = nullptr;
+  const int* rowD = nullptr;
+  const int* colD = nullptr;
+  double dcoef = 0.0;
 };
 
 struct UpdateParams {

@@ -40,7 +40,8 @@ __device__ __forceinline__ int64_t koff(const int* __restrict__ tab, int64_t aff
 
 __device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n, double s) {
   double* p = g.C + (int64_t)g.rowC[m] + g.colC[n];
-  const double v = g.alpha * s;
+  double v = g.alpha * s;
+  if (g.D) v += g.dcoef * g.D[(int64_t)g.rowD[m] + g.colD[n]];
   if (g.accumulate) *p += v; else *p = v;
 }
 
@@ -1118,6 +1119,8 @@ static GemmArgs transposed(const GemmArgs& g) {
   t.kB_aff = g.kA_aff;
   t.rowC = g.colC;
   t.colC = g.rowC;
+  t.rowD = g.colD;
+  t.colD = g.rowD;
   t.a_kfast = g.b_kfast;
   t.b_kfast = g.a_kfast;
   t.trans = 0;
