@@ -751,7 +751,7 @@ struct ccq_hip_ctx {
           (g.trans ? "t" : "");
     else
       t = std::to_string(g.bm) + "x" + std::to_string(g.bn) + "k" + std::to_string(g.bk) +
-          (g.ksplit == 2 ? "d" : "");
+          (g.ksplit == 2 ? "d" : "") + (g.trans ? "t" : "");
     return t + "s" + std::to_string(g.nsplit);
   }
 
@@ -841,6 +841,8 @@ struct ccq_hip_ctx {
                       : (!ns.empty() && stride_of(B, op.lb, ns.back()) == 1 ? 0 : 1));
           g.kA_aff = affine_stride(kA);
           g.kB_aff = affine_stride(kB);
+          g.c_mfast = affine_stride(rC) == 1 ? 1 : 0;
+          g.c_nfast = affine_stride(cC) == 1 ? 1 : 0;
           ccqk::gemm_plan(g, kWsCap);
           ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(g));
           op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +

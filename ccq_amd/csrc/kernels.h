@@ -25,6 +25,8 @@ struct GemmArgs {
   int accumulate;
   int a_kfast;
   int b_kfast;
+  // C is contiguous along m (rowC[m] == m) / along n (colC[n] == n)
+  int c_mfast = 0, c_nfast = 0;
   // kA[k] == k * kA_aff for every k when kA_aff >= 0 (same for B)
   int64_t kA_aff = -1, kB_aff = -1;
   // launch plan (gemm_plan)

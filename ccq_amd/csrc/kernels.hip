@@ -984,6 +984,10 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
   // resident WGs per CU (128x128 uses more than 256 VGPRs: one)
   const Shape shapes[7] = {{128, 128, 1}, {128, 64, 3}, {64, 128, 3}, {64, 64, 4},
                            {64, 32, 6},   {32, 64, 6},  {32, 32, 8}};
+  // the epilogue stores 16 consecutive columns per 16 lanes: when C is
+  // contiguous along m only, run the product transposed (C^T = B^T A^T)
+  const int ttile = (g.c_mfast && !g.c_nfast) ? 1 : 0;
+  const int64_t TM_ = ttile ? g.N : g.M, TN_ = ttile ? g.M : g.N;
   for (int bk : {16, 32}) {
     const int64_t ksteps = cdiv(g.K, bk);
     for (const Shape& sh : shapes) {
@@ -993,7 +997,7 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
       // no tiles for dot-like products with a tiny side and a long K (the
       // register-direct kernel with split-K measured faster)
       if (std::min(g.M, g.N) < 16 && g.K >= 4096) continue;
-      const int64_t tiles = cdiv(g.M, sh.bm) * cdiv(g.N, sh.bn);
+      const int64_t tiles = cdiv(TM_, sh.bm) * cdiv(TN_, sh.bn);
       const double slab = (sh.bm / 32.0) * (sh.bn / 32.0) * (bk / 4.0) * 64.0;
       const int64_t maxsplit = std::max<int64_t>(1, std::min<int64_t>(ksteps / 2, 512));
       for (int ksp = 1; ksp <= 2; ++ksp)
@@ -1019,7 +1023,7 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
         }
         cyc += splitk_cycles(g.M, g.N, split);
         const double before = best;
-        take(cyc, 0, 0, sh.bm, sh.bn, bk, nullptr, split, steps * ksp * bk);
+        take(cyc, 0, ttile, sh.bm, sh.bn, bk, nullptr, split, steps * ksp * bk);
         if (best < before) g.ksplit = ksp;
       }
     }
