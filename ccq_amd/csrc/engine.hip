@@ -13,6 +13,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <set>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -125,6 +126,11 @@ struct Section {
   std::string prog = "sweep";   // program the section belongs to
   enum { Compute, Update, Copy, Cct3 } kind = Compute;
   std::vector<Op> ops;
+  // launch order of a compute section: indices into ops, one launch each;
+  // a launch of several indices is a grouped GEMM launch.  Empty: ops in
+  // order, one launch each.
+  std::vector<std::vector<int>> launches;
+  std::vector<std::string> launch_cls;
   UpdateStep upd;
   std::string dst, src;
 };
@@ -870,6 +876,7 @@ struct ccq_hip_ctx {
       }
     }
     fuse_adds(push32);
+    group_gemms();
     ws.alloc(std::max<int64_t>(ws_need, 1));
     if (lsq_n) lsq.alloc(lsq_n);
     HIPCHK(hipMalloc(&tables32, sizeof(int) * std::max<size_t>(host32.size(), 1)));
@@ -954,8 +961,198 @@ struct ccq_hip_ctx {
     }
   }
 
+  // Grouped GEMM launches.  The ops of a compute section are placed on
+  // levels: an op's level is one more than the highest level of the earlier
+  // ops it depends on through memory (it reads what they write, or writes
+  // what they read or write; tensors compared by address range).  Ops keep
+  // their relative order inside a level, so every result is the one of the
+  // sequential order.  On each level, tile GEMMs whose own tiles leave CUs
+  // idle and that share a kernel instantiation run as one launch without
+  // split-K.  CCQ_HIP_GROUP=0 keeps one launch per op in program order.
+  bool group = true;
+  static constexpr int64_t kGroupTiles = 512;   // below this a GEMM joins groups
+  static constexpr int64_t kGroupMinK = 256;
+
+  std::pair<uintptr_t, uintptr_t> span(int t) const {
+    const Tensor& T = tensors[t];
+    int64_t n = 1;
+    for (int i = 0; i < T.rank; ++i) n *= T.dims[i];
+    const uintptr_t b = reinterpret_cast<uintptr_t>(T.ptr);
+    return {b, b + sizeof(double) * (uintptr_t)std::max<int64_t>(n, 1)};
+  }
+
+  // the op's plan as a member of a grouped launch: its tile, full K in
+  // 32-deep slabs, no split
+  static ccqk::GemmArgs group_plan(const Op& op) {
+    ccqk::GemmArgs g = op.plan;
+    g.nsplit = 1;
+    g.kchunk = g.K;
+    if (g.kind == 0 && g.ksplit == 1) g.bk = 32;
+    return g;
+  }
+
+  void group_gemms() {
+    if (!group) return;
+    for (auto& sec : sections) {
+      if (sec.kind != Section::Compute) continue;
+      std::vector<int> idx;
+      for (int i = 0; i < (int)sec.ops.size(); ++i)
+        if (!sec.ops[i].fused) idx.push_back(i);
+      const int n = (int)idx.size();
+      std::vector<std::vector<std::pair<uintptr_t, uintptr_t>>> rd(n), wr(n);
+      for (int q = 0; q < n; ++q) {
+        const Op& op = sec.ops[idx[q]];
+        rd[q].push_back(span(op.a));
+        if (op.gemm) rd[q].push_back(span(op.b));
+        if (op.fuse_a >= 0) rd[q].push_back(span(op.fuse_a));
+        if (!op.assign) rd[q].push_back(span(op.out));
+        wr[q].push_back(span(op.out));
+      }
+      auto meet = [](const std::vector<std::pair<uintptr_t, uintptr_t>>& x,
+                     const std::vector<std::pair<uintptr_t, uintptr_t>>& y) {
+        for (auto& u : x)
+          for (auto& v : y)
+            if (u.first < v.second && v.first < u.second) return true;
+        return false;
+      };
+      std::vector<int> level(n, 0);
+      int top = 0;
+      for (int q = 0; q < n; ++q) {
+        for (int r = 0; r < q; ++r)
+          if (level[r] >= level[q] &&
+              (meet(wr[r], rd[q]) || meet(wr[r], wr[q]) || meet(rd[r], wr[q])))
+            level[q] = level[r] + 1;
+        top = std::max(top, level[q]);
+      }
+      for (int lv = 0; lv <= top; ++lv) {
+        std::vector<int> here;
+        for (int q = 0; q < n; ++q)
+          if (level[q] == lv) here.push_back(idx[q]);
+        // candidates by kernel instantiation
+        std::map<int64_t, std::vector<int>> cand;
+        for (int i : here) {
+          const Op& op = sec.ops[i];
+          if (!op.gemm || op.plan.kind != 0 || op.K < kGroupMinK) continue;
+          const ccqk::GemmArgs g = group_plan(op);
+          if (ccqk::gemm_tiles(g) >= kGroupTiles) continue;
+          const int64_t key = ccqk::gemm_group_key(g);
+          if (key >= 0) cand[key].push_back(i);
+        }
+        // a launch lasts as long as its longest workgroup: members of one
+        // group have K within a factor 1.5 (runs of the K-sorted candidates)
+        std::vector<std::vector<int>> runs;
+        for (auto& kv : cand) {
+          auto v = kv.second;
+          std::stable_sort(v.begin(), v.end(),
+                           [&](int x, int y) { return sec.ops[x].K < sec.ops[y].K; });
+          for (size_t a = 0; a < v.size();) {
+            size_t b = a + 1;
+            while (b < v.size() && 2 * sec.ops[v[b]].K <= 3 * sec.ops[v[a]].K) ++b;
+            std::vector<int> run(v.begin() + a, v.begin() + b);
+            std::sort(run.begin(), run.end());
+            runs.push_back(run);
+            a = b;
+          }
+        }
+        std::map<int, std::vector<int>> group_at;   // first member -> members
+        std::set<int> grouped;
+        for (const auto& v : runs) {
+          int64_t tiles = 0;
+          for (int i : v) tiles += ccqk::gemm_tiles(group_plan(sec.ops[i]));
+          if (v.size() < 2 || tiles < 256) continue;
+          const int nch = (int)((v.size() + ccqk::kMaxGroup - 1) / ccqk::kMaxGroup);
+          for (int c = 0; c < nch; ++c) {
+            const size_t lo = v.size() * c / nch, hi = v.size() * (c + 1) / nch;
+            if (hi - lo < 2) continue;
+            std::vector<int> members(v.begin() + lo, v.begin() + hi);
+            group_at[members[0]] = members;
+            grouped.insert(members.begin(), members.end());
+          }
+        }
+        // this level in op order, each group where its first member was
+        for (int i : here) {
+          auto it = group_at.find(i);
+          if (it != group_at.end()) {
+            sec.launches.push_back(it->second);
+            sec.launch_cls.push_back(group_cls(sec, it->second));
+          } else if (!grouped.count(i)) {
+            sec.launches.push_back({i});
+            sec.launch_cls.push_back(sec.ops[i].cls);
+          }
+        }
+      }
+    }
+  }
+
+  std::string group_cls(const Section& sec, const std::vector<int>& members) const {
+    std::map<std::string, int> shapes;
+    for (int i : members) {
+      const Op& op = sec.ops[i];
+      shapes[std::to_string(op.M) + "x" + std::to_string(op.N) + "x" + std::to_string(op.K)]++;
+    }
+    std::string t = "gemm:";
+    bool first = true;
+    for (auto& kv : shapes) {
+      t += (first ? "" : "+") + std::to_string(kv.second) + "*" + kv.first;
+      first = false;
+    }
+    const ccqk::GemmArgs g = group_plan(sec.ops[members[0]]);
+    return t + "/" + std::to_string(g.bm) + "x" + std::to_string(g.bn) + "k" + std::to_string(g.bk) +
+           (g.trans ? "t" : "") + "g";
+  }
+
   // -------------------------------------------------------------------------
   // execution
+
+  ccqk::GemmArgs gemm_args(const Op& op) const {
+    const Tensor& C = tensors[op.out];
+    const Tensor& A = tensors[op.a];
+    const Tensor& B = tensors[op.b];
+    ccqk::GemmArgs g = op.plan;
+    g.A = A.ptr;
+    g.rowA = tables32 + op.t_rowA;
+    g.kA = tables32 + op.t_kA;
+    g.B = B.ptr;
+    g.kB = tables32 + op.t_kB;
+    g.colB = tables32 + op.t_colB;
+    g.C = C.ptr;
+    g.rowC = tables32 + op.t_rowC;
+    g.colC = tables32 + op.t_colC;
+    g.alpha = op.coef;
+    g.accumulate = op.assign ? 0 : 1;
+    g.ws = ws.p;
+    g.prec = prec;
+    if (op.fuse_a >= 0) {
+      g.D = tensors[op.fuse_a].ptr;
+      g.rowD = tables32 + op.t_rowD;
+      g.colD = tables32 + op.t_colD;
+      g.dcoef = op.dcoef;
+    }
+    return g;
+  }
+
+  void run_group(const Section& sec, const std::vector<int>& members) {
+    ccqk::GemmArgs gs[ccqk::kMaxGroup];
+    const int n = (int)members.size();
+    if (n > ccqk::kMaxGroup) throw Fail("grouped launch too large");
+    int64_t total = 0;
+    for (int k = 0; k < n; ++k) total += ccqk::gemm_tiles(group_plan(sec.ops[members[k]]));
+    for (int k = 0; k < n; ++k) {
+      const Op& op = sec.ops[members[k]];
+      const ccqk::GemmArgs gp = group_plan(op);
+      gs[k] = gemm_args(op);
+      gs[k].nsplit = gp.nsplit;
+      gs[k].kchunk = gp.kchunk;
+      gs[k].bk = gp.bk;
+      // tile-row group of the XCD-aware order: about square blocks of the
+      // member's tiles that one XCD receives (an eighth of the launch)
+      const int64_t tiles = ccqk::gemm_tiles(gs[k]);
+      const int64_t tm = ((gs[k].trans ? op.N : op.M) + gs[k].bm - 1) / gs[k].bm;
+      const double per_xcd = std::max(1.0, std::min((double)tiles, (double)total / 8.0));
+      gs[k].group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, std::lround(std::sqrt(per_xcd))));
+    }
+    HIPCHK(ccqk::launch_gemm_group(stream, gs, n));
+  }
 
   void run_op(const Op& op) {
     if (op.fused) return;
@@ -963,28 +1160,7 @@ struct ccq_hip_ctx {
     const Tensor& A = tensors[op.a];
     const int acc = op.assign ? 0 : 1;
     if (op.gemm) {
-      const Tensor& B = tensors[op.b];
-      ccqk::GemmArgs g = op.plan;
-      g.A = A.ptr;
-      g.rowA = tables32 + op.t_rowA;
-      g.kA = tables32 + op.t_kA;
-      g.B = B.ptr;
-      g.kB = tables32 + op.t_kB;
-      g.colB = tables32 + op.t_colB;
-      g.C = C.ptr;
-      g.rowC = tables32 + op.t_rowC;
-      g.colC = tables32 + op.t_colC;
-      g.alpha = op.coef;
-      g.accumulate = acc;
-      g.ws = ws.p;
-      g.prec = prec;
-      if (op.fuse_a >= 0) {
-        g.D = tensors[op.fuse_a].ptr;
-        g.rowD = tables32 + op.t_rowD;
-        g.colD = tables32 + op.t_colD;
-        g.dcoef = op.dcoef;
-      }
-      HIPCHK(ccqk::launch_gemm(stream, g));
+      HIPCHK(ccqk::launch_gemm(stream, gemm_args(op)));
     } else {
       const bool aligned =
           ((reinterpret_cast<uintptr_t>(A.ptr) | reinterpret_cast<uintptr_t>(C.ptr)) & 15) == 0;
@@ -1143,8 +1319,22 @@ struct ccq_hip_ctx {
       if (sec.kind == Section::Cct3) {
         timed("cct3:" + sec.upd.block, 0, 0, [&] { run_cct3(sec.upd); });
       } else if (sec.kind == Section::Compute) {
-        for (auto& op : sec.ops)
-          if (!op.fused) timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
+        if (sec.launches.empty()) {
+          for (auto& op : sec.ops)
+            if (!op.fused) timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
+        } else {
+          for (size_t k = 0; k < sec.launches.size(); ++k) {
+            const auto& L = sec.launches[k];
+            if (L.size() == 1) {
+              const Op& op = sec.ops[L[0]];
+              timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
+              continue;
+            }
+            double fl = 0, by = 0;
+            for (int i : L) fl += sec.ops[i].flops, by += sec.ops[i].bytes;
+            timed(sec.launch_cls[k], fl, by, [&] { run_group(sec, L); });
+          }
+        }
       } else if (sec.kind == Section::Update) {
         const Tensor& R = tensors[sec.upd.resid];
         double n = 1;
@@ -1646,6 +1836,7 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     ctx->device = (int)opts->device;
     if (const char* g = std::getenv("CCQ_HIP_GRAPH")) ctx->use_graph = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_FUSE")) ctx->fuse = std::string(g) != "0";
+    if (const char* g = std::getenv("CCQ_HIP_GROUP")) ctx->group = std::string(g) != "0";
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->layout();

@@ -123,6 +123,22 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap);
 int64_t gemm_tiles(const GemmArgs& g);
 int64_t gemm_ws_need(const GemmArgs& g);
 hipError_t launch_gemm(hipStream_t s, const GemmArgs& g);
+
+// Several independent GEMMs in one launch of the LDS-tiled kernel: one grid
+// covers the tiles of every member (full K each, no split-K), so products
+// whose own tiles would leave CUs idle fill the chip together.  Members must
+// share the kernel instantiation (gemm_group_key) and must not write
+// anything another member reads or writes.
+constexpr int kMaxGroup = 8;
+struct GemmGroup {
+  GemmArgs g[kMaxGroup];
+  int start[kMaxGroup + 1];   // first logical tile of each member
+  int n;
+};
+// kernel instantiation of a plan (after transposition), or -1 when the plan
+// cannot join a group (register kernel, split-K)
+int64_t gemm_group_key(const GemmArgs& g);
+hipError_t launch_gemm_group(hipStream_t s, const GemmArgs* gs, int n);
 hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
                            const int* colA, double* C, const int* rowC, const int* colC,
                            double alpha, int accumulate);

@@ -69,13 +69,17 @@ __device__ __forceinline__ double sum_strided(const double* __restrict__ q, int6
 // logical tiles, and logical tiles are ordered in groups of `group` tile rows
 // (all of one split-K slice first), so the A row panels and B column panels a
 // workgroup needs are mostly shared with the other workgroups on its XCD.
+__device__ __forceinline__ int xcd_logical(int lin, int total) {
+  const int q = total >> 3, r = total & 7;
+  const int xcd = lin & 7, slot = lin >> 3;
+  return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
 __device__ __forceinline__ void tile_coords(int group, int& tx, int& ty, int& tz) {
   const int tm = gridDim.x, tn = gridDim.y;
   const int total = tm * tn * (int)gridDim.z;
   const int lin = blockIdx.x + tm * (blockIdx.y + tn * blockIdx.z);
-  const int q = total >> 3, r = total & 7;
-  const int xcd = lin & 7, slot = lin >> 3;
-  const int logical = xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+  const int logical = xcd_logical(lin, total);
   const int per_z = tm * tn;
   tz = logical / per_z;
   const int pid = logical - tz * per_z;
@@ -85,6 +89,30 @@ __device__ __forceinline__ void tile_coords(int group, int& tx, int& ty, int& tz
   const int in = pid - (pid / gsize) * gsize;
   tx = first + in % gm;
   ty = in / gm;
+}
+
+// Grouped launch (1-D grid over the tiles of every member): the member owning
+// the workgroup's logical tile, and the tile inside that member in the order
+// above.  Members occupy consecutive logical ranges, so an XCD's range covers
+// the tiles of one or two members.
+template <int BM, int BN>
+__device__ __forceinline__ int group_tile(const GemmGroup& gg, int& tx, int& ty) {
+  const int logical = xcd_logical((int)blockIdx.x, gg.start[gg.n]);
+  int mem = 0;
+#pragma unroll
+  for (int i = 1; i < kMaxGroup; ++i)
+    if (i < gg.n && logical >= gg.start[i]) mem = i;
+  const GemmArgs& g = gg.g[mem];
+  const int tm = (int)((g.M + BM - 1) / BM), tn = (int)((g.N + BN - 1) / BN);
+  const int pid = logical - gg.start[mem];
+  const int group = g.group;
+  const int gsize = group * tn;
+  const int first = (pid / gsize) * group;
+  const int gm = tm - first < group ? tm - first : group;
+  const int in = pid - (pid / gsize) * gsize;
+  tx = first + in % gm;
+  ty = in / gm;
+  return mem;
 }
 
 // Split-K: workgroup z of a tile stores its partial product of K range
@@ -130,19 +158,25 @@ struct Loader {
       row_off[j] = g < R_total ? rowtab[g] : -1;
     }
   }
+  // K offsets: k * kaff when affine, else the workgroup's copy of the table
+  // in LDS (lk[k - kbase]) when use_l, else the table in global memory
   __device__ void fetch(double (&reg)[L], const double* __restrict__ X, const int* __restrict__ ktab,
-                        int64_t kaff, int64_t k0, int64_t kend) const {
+                        int64_t kaff, const int* lk, bool use_l, int64_t kbase, int64_t k0,
+                        int64_t kend) const {
+    auto ko = [&](int64_t gk) -> int64_t {
+      return kaff >= 0 ? gk * kaff : (use_l ? (int64_t)lk[gk - kbase] : (int64_t)ktab[gk]);
+    };
     if (KF) {
       const int64_t gk = k0 + kk[0];
       const bool kin = gk < kend;
-      const int64_t ko = kin ? koff(ktab, kaff, gk) : 0;
+      const int64_t o = kin ? ko(gk) : 0;
 #pragma unroll
-      for (int j = 0; j < L; ++j) reg[j] = (kin && row_off[j] >= 0) ? X[(int64_t)row_off[j] + ko] : 0.0;
+      for (int j = 0; j < L; ++j) reg[j] = (kin && row_off[j] >= 0) ? X[(int64_t)row_off[j] + o] : 0.0;
     } else {
 #pragma unroll
       for (int j = 0; j < L; ++j) {
         const int64_t gk = k0 + kk[j];
-        reg[j] = (gk < kend && row_off[j] >= 0) ? X[(int64_t)row_off[j] + koff(ktab, kaff, gk)] : 0.0;
+        reg[j] = (gk < kend && row_off[j] >= 0) ? X[(int64_t)row_off[j] + ko(gk)] : 0.0;
       }
     }
   }
@@ -191,25 +225,72 @@ struct Mma<float> {
 // combine their accumulators through LDS at the end in a fixed order -- the
 // K split of a global split-K without its partial products in HBM and its
 // reduction launch.
+// LDS bytes of one tile-kernel variant
+template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+struct TabLds {
+  static constexpr int tile = KS * (LdsTile<T, BM, AKF, BK>::size + LdsTile<T, BN, BKF, BK>::size) * (int)sizeof(T);
+  static constexpr int red = (KS - 1) * 256 * (BM / 32) * (BN / 32) * 4 * (int)sizeof(T);
+  static constexpr int bytes = tile > red ? tile : red;
+};
+
+// K-offset tables of a workgroup's K range are copied into LDS when the range
+// has at most kKCache entries (the non-k-contiguous loader reads L offsets
+// per slab; from global memory each is a dependent load before the data)
+constexpr int kKCache = 2048;
+
+template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+__device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty, int tz, unsigned char* smem,
+                                              int* kts);
+
 template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
 __global__ __launch_bounds__(256 * KS, (KS == 1 && !(sizeof(T) == 8 && BM * BN >= 128 * 128)) ? 2 : 1)
 void gemm_tab(GemmArgs g) {
+  __shared__ __align__(16) unsigned char smem[TabLds<T, BM, BN, BK, AKF, BKF, KS>::bytes];
+  __shared__ int kts[2 * kKCache];
+  int tx, ty, tz;
+  tile_coords(g.group, tx, ty, tz);
+  gemm_tab_tile<T, BM, BN, BK, AKF, BKF, KS>(g, tx, ty, tz, smem, kts);
+}
+
+// The same tile kernel over the tiles of a group of GEMMs (full K each).  The
+// operand layouts (k-contiguous or not) may differ between members: the
+// workgroup branches once, uniformly, to the body of its member's layouts.
+template <typename T, int BM, int BN, int BK>
+__global__ __launch_bounds__(256, !(sizeof(T) == 8 && BM * BN >= 128 * 128) ? 2 : 1)
+void gemm_tab_group(GemmGroup gg) {
+  constexpr int b11 = TabLds<T, BM, BN, BK, true, true, 1>::bytes;
+  constexpr int b10 = TabLds<T, BM, BN, BK, true, false, 1>::bytes;
+  constexpr int b01 = TabLds<T, BM, BN, BK, false, true, 1>::bytes;
+  constexpr int b00 = TabLds<T, BM, BN, BK, false, false, 1>::bytes;
+  constexpr int m1 = b11 > b10 ? b11 : b10, m2 = b01 > b00 ? b01 : b00;
+  __shared__ __align__(16) unsigned char smem[m1 > m2 ? m1 : m2];
+  __shared__ int kts[2 * kKCache];
+  int tx, ty;
+  const int mem = group_tile<BM, BN>(gg, tx, ty);
+  const GemmArgs& g = gg.g[mem];
+  if (g.a_kfast) {
+    if (g.b_kfast) gemm_tab_tile<T, BM, BN, BK, true, true, 1>(g, tx, ty, 0, smem, kts);
+    else gemm_tab_tile<T, BM, BN, BK, true, false, 1>(g, tx, ty, 0, smem, kts);
+  } else {
+    if (g.b_kfast) gemm_tab_tile<T, BM, BN, BK, false, true, 1>(g, tx, ty, 0, smem, kts);
+    else gemm_tab_tile<T, BM, BN, BK, false, false, 1>(g, tx, ty, 0, smem, kts);
+  }
+}
+
+template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+__device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty, int tz, unsigned char* smem,
+                                              int* kts) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   using TA = LdsTile<T, BM, AKF, BK>;
   using TB = LdsTile<T, BN, BKF, BK>;
   using MM = Mma<T>;
-  constexpr int TILE_BYTES = KS * (TA::size + TB::size) * (int)sizeof(T);
-  constexpr int RED_BYTES = (KS - 1) * 256 * TM * TN * 4 * (int)sizeof(T);
-  __shared__ __align__(16) unsigned char smem[TILE_BYTES > RED_BYTES ? TILE_BYTES : RED_BYTES];
   const int grp = threadIdx.x >> 8;
   T* As = reinterpret_cast<T*>(smem) + grp * (TA::size + TB::size);
   T* Bs = As + TA::size;
   const int tid = threadIdx.x & 255;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  int tx, ty, tz;
-  tile_coords(g.group, tx, ty, tz);
   const int64_t m0 = (int64_t)tx * BM;
   const int64_t n0 = (int64_t)ty * BN;
   const int64_t kbeg = (int64_t)tz * g.kchunk;
@@ -223,6 +304,18 @@ void gemm_tab(GemmArgs g) {
   lb.init(g.colB, n0, g.N, tid);
   double ra[Loader<BM, AKF, BK>::L], rb[Loader<BN, BKF, BK>::L];
 
+  // non-affine K tables of this K range into LDS (uniform condition)
+  const int64_t klen = kend > kbeg ? kend - kbeg : 0;
+  const bool a_l = g.kA_aff < 0 && klen <= kKCache;
+  const bool b_l = g.kB_aff < 0 && klen <= kKCache;
+  if (a_l || b_l) {
+    for (int i = (int)threadIdx.x; i < (int)klen; i += (int)blockDim.x) {
+      if (a_l) kts[i] = g.kA[kbeg + i];
+      if (b_l) kts[kKCache + i] = g.kB[kbeg + i];
+    }
+    __syncthreads();
+  }
+
   typename MM::acc_t acc[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
@@ -234,8 +327,8 @@ void gemm_tab(GemmArgs g) {
   const int64_t nslabs = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   const int64_t nsteps = (nslabs + KS - 1) / KS;
   if (nsteps > 0) {
-    la.fetch(ra, g.A, g.kA, g.kA_aff, kbeg + grp * BK, kend);
-    lb.fetch(rb, g.B, g.kB, g.kB_aff, kbeg + grp * BK, kend);
+    la.fetch(ra, g.A, g.kA, g.kA_aff, kts, a_l, kbeg, kbeg + grp * BK, kend);
+    lb.fetch(rb, g.B, g.kB, g.kB_aff, kts + kKCache, b_l, kbeg, kbeg + grp * BK, kend);
   }
   for (int64_t s = 0; s < nsteps; ++s) {
     __syncthreads();
@@ -244,8 +337,8 @@ void gemm_tab(GemmArgs g) {
     __syncthreads();
     if (s + 1 < nsteps) {
       const int64_t k0 = kbeg + ((s + 1) * KS + grp) * BK;
-      la.fetch(ra, g.A, g.kA, g.kA_aff, k0, kend);
-      lb.fetch(rb, g.B, g.kB, g.kB_aff, k0, kend);
+      la.fetch(ra, g.A, g.kA, g.kA_aff, kts, a_l, kbeg, k0, kend);
+      lb.fetch(rb, g.B, g.kB, g.kB_aff, kts + kKCache, b_l, kbeg, k0, kend);
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
@@ -1190,6 +1283,54 @@ hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
   hipLaunchKernelGGL(gemm_splitk_reduce,
                      dim3((unsigned)cdiv(g.N, per_block), (unsigned)std::min<int64_t>(g.M, 65535)),
                      dim3(256), 0, s, g, pshift);
+  return hipGetLastError();
+}
+
+// Grouped launches run the KS = 1, BK = 32 tile kernel; members share the
+// tile shape (after transposition) and the arithmetic.
+int64_t gemm_group_key(const GemmArgs& g0) {
+  if (g0.kind != 0 || g0.nsplit != 1 || g0.ksplit != 1 || g0.bk != 32) return -1;
+  const GemmArgs g = g0.trans ? transposed(g0) : g0;
+  return ((int64_t)g.bm * 1000 + g.bn) * 2 + g.prec;
+}
+
+template <int BM, int BN>
+static void launch_group(hipStream_t s, const GemmGroup& gg) {
+  const dim3 grid((unsigned)gg.start[gg.n]), blk(256);
+  if (gg.g[0].prec == 1) hipLaunchKernelGGL((gemm_tab_group<float, BM, BN, 32>), grid, blk, 0, s, gg);
+  else hipLaunchKernelGGL((gemm_tab_group<double, BM, BN, 32>), grid, blk, 0, s, gg);
+}
+
+hipError_t launch_gemm_group(hipStream_t s, const GemmArgs* gs, int n) {
+  if (n < 1 || n > kMaxGroup) return hipErrorInvalidValue;
+  GemmGroup gg{};
+  const int64_t key = gemm_group_key(gs[0]);
+  if (key < 0) return hipErrorInvalidValue;
+  int64_t total = 0;
+  gg.n = 0;
+  for (int i = 0; i < n; ++i) {
+    if (gemm_group_key(gs[i]) != key) return hipErrorInvalidValue;
+    const GemmArgs g = gs[i].trans ? transposed(gs[i]) : gs[i];
+    if (g.M <= 0 || g.N <= 0) continue;
+    gg.g[gg.n] = g;
+    gg.start[gg.n] = (int)total;
+    total += cdiv(g.M, g.bm) * cdiv(g.N, g.bn);
+    if (total > 0x7fffffffLL) return hipErrorInvalidValue;
+    ++gg.n;
+  }
+  if (gg.n == 0) return hipSuccess;
+  gg.start[gg.n] = (int)total;
+  const GemmArgs& g = gg.g[0];
+  switch (g.bm * 1000 + g.bn) {
+    case 128128: launch_group<128, 128>(s, gg); break;
+    case 128064: launch_group<128, 64>(s, gg); break;
+    case 64128: launch_group<64, 128>(s, gg); break;
+    case 64064: launch_group<64, 64>(s, gg); break;
+    case 64032: launch_group<64, 32>(s, gg); break;
+    case 32064: launch_group<32, 64>(s, gg); break;
+    case 32032: launch_group<32, 32>(s, gg); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

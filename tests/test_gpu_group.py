@@ -1,0 +1,97 @@
+"""Grouped GEMM launches (independent tile GEMMs of one dependency level run
+as one launch at full K) against one launch per GEMM in program order
+(CCQ_HIP_GROUP=0), on systems large enough that groups form.
+
+Bar: the profile shows grouped classes; energies after three sweeps agree to
+1e-11 relative and amplitudes to 1e-11 of their largest entry (the two paths
+differ only by the split-K summation order of the ungrouped launches); the
+captured graph of the grouped sweep replays bit for bit as eager launches.
+"""
+
+import re
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import solver as osolver
+from ccq_amd import engine as E
+from gpu_helpers import Case, antisym_random
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _state(c, method, seed, scale=0.02):
+    rng = np.random.default_rng(seed)
+    t = osolver.zero_amplitudes(method, c.ca, c.cb, c.va, c.vb)
+    lvl = {"CCSD": 2, "CCSDT": 3}[method]
+    for k in t:
+        if osolver.BLOCKS[k][0] <= lvl:
+            t[k] = antisym_random(rng, k, c.ca, c.cb, c.va, c.vb, scale)
+    return E.join_tvec(t, c.ca, c.cb, c.va, c.vb)
+
+
+def _engine(c, method, monkeypatch, group, graph="0"):
+    monkeypatch.setenv("CCQ_HIP_GROUP", group)
+    monkeypatch.setenv("CCQ_HIP_GRAPH", graph)
+    eng = E.Engine(0, c.oa, c.ob, c.n, method, shift=0.02)
+    eng.set_ints(c.fa, c.fb, c.vaa, c.vab, c.vbb)
+    return eng
+
+
+GROUPED = re.compile(r"^gemm:.*\*.*g$")
+
+
+@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 64, 8, 8), ("CCSD", 64, 8, 7), ("CCSDT", 30, 6, 6)])
+def test_grouped_launches_match_single(method, n, o, ob, monkeypatch):
+    _say(f"case grouped launches {method} n={n} o={o} ob={ob}")
+    c = Case(n, o, 2 * n, 23, occ_b=ob)
+    vec = _state(c, method, 3)
+    out = []
+    for group in ("0", "1"):
+        eng = _engine(c, method, monkeypatch, group)
+        eng.set_t(vec)
+        eng.profile(True)
+        eng.update()
+        rep = eng.profile_report()
+        eng.profile(False)
+        grouped = [k for k in rep if GROUPED.match(k)]
+        if group == "1":
+            _say("  grouped classes: " + ", ".join(grouped[:6]))
+            assert grouped, sorted(rep)[:20]
+        else:
+            assert not grouped
+        es = [eng.energy()]
+        for _ in range(2):
+            eng.update()
+            es.append(eng.energy())
+        out.append((np.array(es), eng.get_t()))
+        eng.close()
+    (e0, t0), (e1, t1) = out
+    np.testing.assert_allclose(e1, e0, rtol=1e-11, atol=0)
+    assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
+
+
+def test_grouped_graph_replays_equal_eager(monkeypatch):
+    _say("case grouped graph replay")
+    c = Case(64, 8, 128, 29)
+    vec = _state(c, "CCSD", 5)
+    engines = [_engine(c, "CCSD", monkeypatch, "1", graph) for graph in ("1", "0")]
+    for eng in engines:
+        eng.set_t(vec)
+    for rep in range(3):
+        res = []
+        for eng in engines:
+            eng.update()
+            e = eng.energy()
+            eng.diis_record(rep + 1)
+            res.append((e, eng.get_t()))
+        assert res[0][0] == res[1][0]
+        np.testing.assert_array_equal(res[0][1], res[1][1])
+    for eng in engines:
+        eng.close()
