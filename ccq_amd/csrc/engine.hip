@@ -84,6 +84,8 @@ struct Tensor {
   // gathered into a dense C-order copy (ptr/strides) after every upload
   double* src = nullptr;
   int64_t src_strides[8] = {0};
+  // position pairs (p < q) in which the block is antisymmetric
+  std::vector<std::pair<int, int>> anti;
 };
 
 struct Op {
@@ -102,6 +104,10 @@ struct Op {
   int fuse_a = -1;
   double dcoef = 0.0;
   int64_t t_rowD = 0, t_colD = 0;
+  // antisymmetric pairs packed to their canonical half: a summed pair (k),
+  // an output pair from A (m, mirrored rows rowC2) or from B (n, colC2)
+  bool pack_k = false;
+  int64_t t_rowC2 = -1, t_colC2 = -1;
   double flops = 0, bytes = 0;
   std::string cls;           // profile class
 };
@@ -521,6 +527,18 @@ struct ccq_hip_ctx {
           local[name] = (int)tensors.size();
           tensors.push_back(t);
         }
+      } else if (kw == "anti") {
+        std::string id, pq;
+        ls >> id;
+        Tensor& t = tensors[local.at(id)];
+        while (ls >> pq) {
+          const auto c = pq.find(',');
+          if (c == std::string::npos) throw Fail("bad anti record for " + id);
+          const int p = std::stoi(pq.substr(0, c)), q = std::stoi(pq.substr(c + 1));
+          if (p < 0 || q <= p || q >= t.rank || t.dims[p] != t.dims[q]) throw Fail("bad anti pair for " + id);
+          if (std::find(t.anti.begin(), t.anti.end(), std::make_pair(p, q)) == t.anti.end())
+            t.anti.emplace_back(p, q);
+        }
       } else if (kw == "section") {
         sections.emplace_back();
         cur = &sections.back();
@@ -706,8 +724,13 @@ struct ccq_hip_ctx {
     HIPCHK(hipStreamSynchronize(stream));
   }
 
+  // Offsets of the index combinations of `subset` (row-major in subset
+  // order).  With a pair (px < py, positions in subset), only combinations
+  // with idx[px] < idx[py] are listed; swap gives the offsets of the mirrored
+  // element (the two indices exchanged).
   static std::vector<int64_t> offsets(const Tensor& t, const std::string& labels,
-                                      const std::string& subset) {
+                                      const std::string& subset, int px = -1, int py = -1,
+                                      bool swap = false) {
     std::vector<int64_t> ext, str;
     for (char c : subset) {
       auto pos = labels.find(c);
@@ -715,22 +738,50 @@ struct ccq_hip_ctx {
       ext.push_back(t.dims[pos]);
       str.push_back(t.strides[pos]);
     }
+    if (swap) std::swap(str[px], str[py]);
     int64_t total = 1;
     for (auto e : ext) total *= e;
-    std::vector<int64_t> out((size_t)std::max<int64_t>(total, 1), 0);
-    if (subset.empty()) return out;
+    std::vector<int64_t> out;
+    out.reserve((size_t)std::max<int64_t>(total, 1));
+    if (subset.empty()) return std::vector<int64_t>(1, 0);
     std::vector<int64_t> idx(ext.size(), 0);
     for (int64_t n = 0; n < total; ++n) {
-      int64_t o = 0;
-      for (size_t k = 0; k < ext.size(); ++k) o += idx[k] * str[k];
-      out[n] = o;
+      if (px < 0 || idx[px] < idx[py]) {
+        int64_t o = 0;
+        for (size_t k = 0; k < ext.size(); ++k) o += idx[k] * str[k];
+        out.push_back(o);
+      }
       for (int k = (int)ext.size() - 1; k >= 0; --k) {
         if (++idx[k] < ext[k]) break;
         idx[k] = 0;
       }
     }
+    if (out.empty()) out.push_back(0);
     return out;
   }
+
+  static bool is_anti(const Tensor& t, const std::string& labels, char x, char y) {
+    const auto p = labels.find(x), q = labels.find(y);
+    if (p == std::string::npos || q == std::string::npos) return false;
+    const auto pr = std::make_pair((int)std::min(p, q), (int)std::max(p, q));
+    return std::find(t.anti.begin(), t.anti.end(), pr) != t.anti.end();
+  }
+
+  // first pair of labels of `sub` (extent at least 2) antisymmetric in both
+  // (X, lx) and, when given, (Y, ly): positions in sub, or {-1, -1}
+  static std::pair<int, int> find_pair(const std::string& sub, const Tensor& X, const std::string& lx,
+                                       const Tensor* Y, const std::string& ly) {
+    for (int i = 0; i < (int)sub.size(); ++i)
+      for (int j = i + 1; j < (int)sub.size(); ++j)
+        if (X.dims[lx.find(sub[i])] >= 2 && is_anti(X, lx, sub[i], sub[j]) &&
+            (!Y || is_anti(*Y, ly, sub[i], sub[j])))
+          return {i, j};
+    return {-1, -1};
+  }
+
+  // CCQ_HIP_PACK=0 computes every element of antisymmetric pairs
+  bool pack_pairs = true;
+  static constexpr int64_t kPackMinK = 64;
 
   static int64_t stride_of(const Tensor& t, const std::string& labels, char c) {
     auto pos = labels.find(c);
@@ -825,9 +876,31 @@ struct ccq_hip_ctx {
             auto pa = op.la.find(c), pb = op.lb.find(c);
             if (A.dims[pa] != B.dims[pb]) throw Fail("extent mismatch on summed label");
           }
-          auto rA = offsets(A, op.la, ms), kA = offsets(A, op.la, ks);
-          auto kB = offsets(B, op.lb, ks), cB = offsets(B, op.lb, ns);
-          auto rC = offsets(C, op.lc, ms), cC = offsets(C, op.lc, ns);
+          // antisymmetric pairs: a summed pair antisymmetric in A and B
+          // contributes twice its canonical half; an output pair carried
+          // by A (or B) makes the contribution antisymmetric in it, so only
+          // its canonical half is computed and the mirrored elements are
+          // written with the opposite sign (accumulating ops only: the
+          // diagonal is never written)
+          // (output pairs only for products with K >= kPackMinK: with a short
+          // K the product is store bound and the mirrored stores cost more
+          // than the halved arithmetic saves)
+          std::pair<int, int> pk{-1, -1}, pm{-1, -1}, pn{-1, -1};
+          if (pack_pairs) {
+            pk = find_pair(ks, A, op.la, &B, op.lb);
+            int64_t kfull = 1;
+            for (char c : ks) kfull *= A.dims[op.la.find(c)];
+            if (!op.assign && kfull >= kPackMinK) {
+              pm = find_pair(ms, A, op.la, nullptr, "");
+              pn = find_pair(ns, B, op.lb, nullptr, "");
+            }
+          }
+          auto rA = offsets(A, op.la, ms, pm.first, pm.second);
+          auto kA = offsets(A, op.la, ks, pk.first, pk.second);
+          auto kB = offsets(B, op.lb, ks, pk.first, pk.second);
+          auto cB = offsets(B, op.lb, ns, pn.first, pn.second);
+          auto rC = offsets(C, op.lc, ms, pm.first, pm.second);
+          auto cC = offsets(C, op.lc, ns, pn.first, pn.second);
           op.M = (int64_t)rA.size();
           op.N = (int64_t)cB.size();
           op.K = (int64_t)kA.size();
@@ -837,6 +910,10 @@ struct ccq_hip_ctx {
           op.t_colB = push32(cB);
           op.t_rowC = push32(rC);
           op.t_colC = push32(cC);
+          op.pack_k = pk.first >= 0;
+          if (pk.first >= 0) op.coef *= 2.0;
+          if (pm.first >= 0) op.t_rowC2 = push32(offsets(C, op.lc, ms, pm.first, pm.second, true));
+          if (pn.first >= 0) op.t_colC2 = push32(offsets(C, op.lc, ns, pn.first, pn.second, true));
           ccqk::GemmArgs& g = op.plan;
           g.M = op.M;
           g.N = op.N;
@@ -852,7 +929,8 @@ struct ccq_hip_ctx {
           ccqk::gemm_plan(g, kWsCap);
           ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(g));
           op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +
-                   std::to_string(op.K) + "/" + plan_tag(g);
+                   std::to_string(op.K) + "/" + plan_tag(g) +
+                   (op.pack_k || op.t_rowC2 >= 0 || op.t_colC2 >= 0 ? "p" : "");
           op.flops = 2.0 * (double)op.M * (double)op.N * (double)(ks.empty() ? 1 : op.K);
           op.bytes = 8.0 * ((double)op.M * op.K + (double)op.K * op.N + 2.0 * op.M * op.N);
         } else {
@@ -913,7 +991,7 @@ struct ccq_hip_ctx {
         if (j == ops.size()) continue;
         Op& gm = ops[j];
         if (!gm.gemm || gm.out != ad.out || gm.assign || gm.fuse_a >= 0 || gm.a == ad.out ||
-            gm.b == ad.out)
+            gm.b == ad.out || gm.t_rowC2 >= 0 || gm.t_colC2 >= 0)
           continue;
         bool clobbered = false;
         for (size_t k = i + 1; k < j; ++k)
@@ -1097,8 +1175,13 @@ struct ccq_hip_ctx {
       first = false;
     }
     const ccqk::GemmArgs g = group_plan(sec.ops[members[0]]);
+    bool packed = false;
+    for (int i : members) {
+      const Op& op = sec.ops[i];
+      packed = packed || op.pack_k || op.t_rowC2 >= 0 || op.t_colC2 >= 0;
+    }
     return t + "/" + std::to_string(g.bm) + "x" + std::to_string(g.bn) + "k" + std::to_string(g.bk) +
-           (g.trans ? "t" : "") + "g";
+           (g.trans ? "t" : "") + (packed ? "p" : "") + "g";
   }
 
   // -------------------------------------------------------------------------
@@ -1128,6 +1211,8 @@ struct ccq_hip_ctx {
       g.colD = tables32 + op.t_colD;
       g.dcoef = op.dcoef;
     }
+    if (op.t_rowC2 >= 0) g.rowC2 = tables32 + op.t_rowC2;
+    if (op.t_colC2 >= 0) g.colC2 = tables32 + op.t_colC2;
     return g;
   }
 
@@ -1837,6 +1922,7 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (const char* g = std::getenv("CCQ_HIP_GRAPH")) ctx->use_graph = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_FUSE")) ctx->fuse = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_GROUP")) ctx->group = std::string(g) != "0";
+    if (const char* g = std::getenv("CCQ_HIP_PACK")) ctx->pack_pairs = std::string(g) != "0";
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->layout();

@@ -51,6 +51,11 @@ struct GemmArgs {
   const int* rowD = nullptr;
   const int* colD = nullptr;
   double dcoef = 0.0;
+  // mirrored images of an output antisymmetric in a packed pair: the value
+  // is also stored, negated, at rowC2[m] + colC[n] and rowC[m] + colC2[n]
+  // (and at rowC2[m] + colC2[n] unnegated when both are set)
+  const int* rowC2 = nullptr;
+  const int* colC2 = nullptr;
 };
 
 struct UpdateParams {

@@ -38,11 +38,21 @@ __device__ __forceinline__ int64_t koff(const int* __restrict__ tab, int64_t aff
   return aff >= 0 ? k * aff : (int64_t)tab[k];
 }
 
+__device__ __forceinline__ void put_c(double* p, double v, int accumulate) {
+  if (accumulate) *p += v; else *p = v;
+}
+
 __device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n, double s) {
-  double* p = g.C + (int64_t)g.rowC[m] + g.colC[n];
+  const int64_t r = g.rowC[m], c = g.colC[n];
   double v = g.alpha * s;
   if (g.D) v += g.dcoef * g.D[(int64_t)g.rowD[m] + g.colD[n]];
-  if (g.accumulate) *p += v; else *p = v;
+  put_c(g.C + r + c, v, g.accumulate);
+  if (g.rowC2) {
+    const int64_t r2 = g.rowC2[m];
+    put_c(g.C + r2 + c, -v, g.accumulate);
+    if (g.colC2) put_c(g.C + r2 + g.colC2[n], v, g.accumulate);
+  }
+  if (g.colC2) put_c(g.C + r + g.colC2[n], -v, g.accumulate);
 }
 
 // sum_{i < cnt} q[i * stride] with four interleaved partial sums (fixed order)
@@ -1218,6 +1228,8 @@ static GemmArgs transposed(const GemmArgs& g) {
   t.colC = g.rowC;
   t.rowD = g.colD;
   t.colD = g.rowD;
+  t.rowC2 = g.colC2;
+  t.colC2 = g.rowC2;
   t.a_kfast = g.b_kfast;
   t.b_kfast = g.a_kfast;
   t.trans = 0;

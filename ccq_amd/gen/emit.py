@@ -7,6 +7,11 @@ Format (one record per line, '#' comments):
   tensor <id> view <base> <rank> <sp>...     slice of a full-range base array
   tensor <id> amp <block>                    amplitude block (t1a, t2ab, ...)
   tensor <id> resid <block>                  residual accumulator of a block
+  anti <id> <p>,<q>...                       position pairs of equal space and
+                                             spin in which the stored block is
+                                             antisymmetric (not for residual
+                                             accumulators, which hold one
+                                             representative per image set)
   program <name>                             sections that follow belong to
                                              program <name> (default: sweep)
   section <name>
@@ -45,6 +50,18 @@ def _base_of(key):
     raise KeyError(key)
 
 
+def anti_pairs(decl, key):
+    """Position pairs (p < q) of one declared antisymmetric group whose indices
+    have the same space and spin in this block."""
+    out = []
+    for g in decl.groups:
+        for i, p in enumerate(g):
+            for q in g[i + 1:]:
+                if key.spaces[p] == key.spaces[q] and key.spins[p] == key.spins[q]:
+                    out.append((min(p, q), max(p, q)))
+    return sorted(out)
+
+
 def emit(prog, sections, method: str, rhf: bool) -> str:
     ids = {}
     lines = [f"ccq-ir 1 {method} {'rhf' if rhf else 'uhf'}"]
@@ -69,6 +86,10 @@ def emit(prog, sections, method: str, rhf: bool) -> str:
         else:
             dims = " ".join(DIMSYM[(c, s)] for c, s in zip(key.spaces, key.spins))
             lines.append(f"tensor {name} inter {rank} {dims}")
+        if kind != 'resid':
+            pairs = anti_pairs(prog.decls[key.name], key)
+            if pairs:
+                lines.append(f"anti {name} " + " ".join(f"{p},{q}" for p, q in pairs))
         return name
 
     body = []

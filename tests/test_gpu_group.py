@@ -95,3 +95,61 @@ def test_grouped_graph_replays_equal_eager(monkeypatch):
         np.testing.assert_array_equal(res[0][1], res[1][1])
     for eng in engines:
         eng.close()
+
+
+def _engine_env(c, method, monkeypatch, env, **kw):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    eng = E.Engine(0, c.oa, c.ob, c.n, method, **kw)
+    eng.set_ints(c.fa, c.fb, c.vaa, c.vab, c.vbb)
+    return eng
+
+
+@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 16, 4, 3), ("CCSD", 24, 5, 5), ("CCSDT", 10, 3, 2),
+                                           ("CCSDTQ", 8, 3, 2), ("CCSDt", 10, 3, 2)])
+def test_packed_pairs_match_full(method, n, o, ob, monkeypatch):
+    """Antisymmetric pairs packed to their canonical half (summed pairs
+    doubled, output pairs mirrored with the opposite sign) against every
+    element computed (CCQ_HIP_PACK=0): three sweeps agree to 1e-11."""
+    _say(f"case packed pairs {method} n={n} o={o} ob={ob}")
+    c = Case(n, o, 2 * n, 37, occ_b=ob)
+    rng = np.random.default_rng(11)
+    t = osolver.zero_amplitudes("CCSDT", c.ca, c.cb, c.va, c.vb)
+    lvl = {"CCSD": 2, "CCSDT": 3, "CCSDTQ": 3, "CCSDt": 3}[method]
+    for k in t:
+        if osolver.BLOCKS[k][0] <= lvl:
+            t[k] = antisym_random(rng, k, c.ca, c.cb, c.va, c.vb, 0.03)
+    kw = dict(shift=0.03)
+    if method == "CCSDt":
+        from oracle import active as A
+        kw.update(act_occ=1, act_unocc=2)
+        act = A.ActiveSpace(0, c.oa, c.ob, c.n, 1, 2)
+        for k in ("t3aaa", "t3aab", "t3abb", "t3bbb"):
+            t[k][~A.p_mask(k, t[k].shape, act)] = 0.0
+        vec = A.tvec(t, act, c.ca, c.cb, c.va, c.vb)
+    else:
+        if method == "CCSD":
+            for k in ("t3aaa", "t3aab", "t3abb", "t3bbb"):
+                t[k] = np.zeros_like(t[k])
+        vec = E.join_tvec(t, c.ca, c.cb, c.va, c.vb)
+    out = []
+    for pack in ("0", "1"):
+        eng = _engine_env(c, method, monkeypatch, {"CCQ_HIP_PACK": pack, "CCQ_HIP_GRAPH": "0"}, **kw)
+        eng.set_t(vec)
+        eng.profile(True)
+        eng.update()
+        rep = eng.profile_report()
+        eng.profile(False)
+        packed = [k for k in rep if k.startswith("gemm:") and (k.endswith("p") or k.endswith("pg"))]
+        assert bool(packed) == (pack == "1"), sorted(rep)[:12]
+        es = [eng.energy()]
+        for _ in range(2):
+            eng.update()
+            es.append(eng.energy())
+        out.append((np.array(es), eng.get_t(), eng.get_t4() if method == "CCSDTQ" else None))
+        eng.close()
+    (e0, t0, q0), (e1, t1, q1) = out
+    np.testing.assert_allclose(e1, e0, rtol=1e-11, atol=0)
+    assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
+    if q0 is not None:
+        assert np.abs(q1 - q0).max() <= 1e-11 * max(np.abs(q0).max(), 1e-12)
