@@ -108,6 +108,10 @@ struct Op {
   // an output pair from A (m, mirrored rows rowC2) or from B (n, colC2)
   bool pack_k = false;
   int64_t t_rowC2 = -1, t_colC2 = -1;
+  // chain of GEMMs into one output run as one launch: index into the
+  // section's chains (the head is launched, the other members are skipped)
+  int chain = -1;
+  bool chain_head = false;
   double flops = 0, bytes = 0;
   std::string cls;           // profile class
 };
@@ -137,6 +141,11 @@ struct Section {
   // order, one launch each.
   std::vector<std::vector<int>> launches;
   std::vector<std::string> launch_cls;
+  // chains: member op indices (program order), their orientation against the
+  // first member (1: transposed) and the launch plan over the concatenated K
+  std::vector<std::vector<int>> chains;
+  std::vector<std::vector<int>> chain_flip;
+  std::vector<ccqk::GemmArgs> chain_plan;
   UpdateStep upd;
   std::string dst, src;
 };
@@ -954,6 +963,13 @@ struct ccq_hip_ctx {
       }
     }
     fuse_adds(push32);
+    // groups first; products left alone are chained where they can be
+    group_gemms();
+    chain_gemms(host32);
+    for (auto& sec : sections) {
+      sec.launches.clear();
+      sec.launch_cls.clear();
+    }
     group_gemms();
     ws.alloc(std::max<int64_t>(ws_need, 1));
     if (lsq_n) lsq.alloc(lsq_n);
@@ -1039,6 +1055,120 @@ struct ccq_hip_ctx {
     }
   }
 
+  // GEMM chains.  Consecutive GEMMs that accumulate into one output with the
+  // same tile mapping (equal row / column offset tables, or equal after
+  // transposing one product) and that the planner would split along K run as
+  // one product over their concatenated K ranges (gemm_tab_chain): one
+  // launch, one split-K reduction, one pass over the output.  The first
+  // member may assign and carry a folded add; the others accumulate.
+  // CCQ_HIP_CHAIN=0 keeps one launch per GEMM.
+  bool chain = true;
+
+  void chain_gemms(const std::vector<int>& host32) {
+    if (!chain) return;
+    auto same = [&](int64_t a, int64_t b, int64_t n) {
+      return std::equal(host32.begin() + a, host32.begin() + a + n, host32.begin() + b);
+    };
+    for (auto& sec : sections) {
+      if (sec.kind != Section::Compute) continue;
+      auto& ops = sec.ops;
+      // products that run in a grouped launch stay there
+      std::set<int> grouped;
+      for (const auto& L : sec.launches)
+        if (L.size() > 1) grouped.insert(L.begin(), L.end());
+      auto eligible = [&](int i) {
+        const Op& op = ops[i];
+        return op.gemm && !op.fused && !grouped.count(i) && op.plan.kind == 0 && op.t_rowC2 < 0 &&
+               op.t_colC2 < 0 && op.a != op.out && op.b != op.out;
+      };
+      std::vector<int> live;
+      for (int i = 0; i < (int)ops.size(); ++i)
+        if (!ops[i].fused) live.push_back(i);
+      for (size_t p = 0; p < live.size();) {
+        const Op& h = ops[live[p]];
+        if (!eligible(live[p]) || h.plan.nsplit < 2) {
+          ++p;
+          continue;
+        }
+        std::vector<int> mem{live[p]}, flip{0};
+        int64_t ktot = h.K;
+        size_t q = p + 1;
+        for (; q < live.size() && mem.size() < (size_t)ccqk::kMaxChain; ++q) {
+          const Op& x = ops[live[q]];
+          if (!eligible(live[q]) || x.out != h.out || x.assign || x.fuse_a >= 0) break;
+          int f = -1;
+          if (x.M == h.M && x.N == h.N && same(x.t_rowC, h.t_rowC, h.M) && same(x.t_colC, h.t_colC, h.N))
+            f = 0;
+          else if (x.M == h.N && x.N == h.M && same(x.t_rowC, h.t_colC, h.N) && same(x.t_colC, h.t_rowC, h.M))
+            f = 1;
+          if (f < 0) break;
+          mem.push_back(live[q]);
+          flip.push_back(f);
+          ktot += x.K;
+        }
+        if (mem.size() < 2) {
+          ++p;
+          continue;
+        }
+        ccqk::GemmArgs cp = h.plan;
+        cp.K = ktot;
+        cp.kA_aff = cp.kB_aff = -1;
+        ccqk::gemm_plan(cp, kWsCap);
+        if (cp.kind != 0) {
+          ++p;
+          continue;
+        }
+        cp.bk = 32;
+        cp.ksplit = 1;
+        ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(cp));
+        const int c = (int)sec.chains.size();
+        sec.chains.push_back(mem);
+        sec.chain_flip.push_back(flip);
+        sec.chain_plan.push_back(cp);
+        for (int i : mem) ops[i].chain = c;
+        ops[mem[0]].chain_head = true;
+        p = q;
+      }
+    }
+  }
+
+  std::string chain_cls(const Section& sec, int c) const {
+    const auto& mem = sec.chains[c];
+    const ccqk::GemmArgs& cp = sec.chain_plan[c];
+    const Op& h = sec.ops[mem[0]];
+    std::string t = "gemm:" + std::to_string(mem.size()) + "*" + std::to_string(h.M) + "x" +
+                    std::to_string(h.N) + "x" + std::to_string(cp.K / (int64_t)mem.size());
+    return t + "/" + std::to_string(cp.bm) + "x" + std::to_string(cp.bn) + "k32s" +
+           std::to_string(cp.nsplit) + (cp.trans ? "t" : "") + "c";
+  }
+
+  // entries of a chain, oriented like its first member run with trans
+  // (1: the whole chain transposed), tile bm x bn, BK 32
+  void chain_entries(const Section& sec, int c, int trans, int bm, int bn, ccqk::GemmArgs* es) const {
+    const auto& mem = sec.chains[c];
+    for (size_t k = 0; k < mem.size(); ++k) {
+      es[k] = gemm_args(sec.ops[mem[k]]);
+      es[k].kind = 0;
+      es[k].bm = bm;
+      es[k].bn = bn;
+      es[k].bk = 32;
+      es[k].ksplit = 1;
+      es[k].nsplit = 1;
+      es[k].kchunk = es[k].K;
+      es[k].trans = sec.chain_flip[c][k] ^ trans;
+    }
+  }
+
+  void run_chain(const Section& sec, int c) {
+    const ccqk::GemmArgs& cp = sec.chain_plan[c];
+    ccqk::GemmArgs es[ccqk::kMaxChain];
+    chain_entries(sec, c, cp.trans, cp.bm, cp.bn, es);
+    es[0].nsplit = cp.nsplit;
+    es[0].kchunk = cp.kchunk;
+    es[0].group = cp.group;
+    HIPCHK(ccqk::launch_gemm_chain(stream, es, (int)sec.chains[c].size()));
+  }
+
   // Grouped GEMM launches.  The ops of a compute section are placed on
   // levels: an op's level is one more than the highest level of the earlier
   // ops it depends on through memory (it reads what they write, or writes
@@ -1048,6 +1178,7 @@ struct ccq_hip_ctx {
   // idle and that share a kernel instantiation run as one launch without
   // split-K.  CCQ_HIP_GROUP=0 keeps one launch per op in program order.
   bool group = true;
+  bool dump_schedule = false;   // CCQ_HIP_SCHEDULE=1: levels and launches to stderr
   static constexpr int64_t kGroupTiles = 512;   // below this a GEMM joins groups
   static constexpr int64_t kGroupMinK = 256;
 
@@ -1070,21 +1201,33 @@ struct ccq_hip_ctx {
   }
 
   void group_gemms() {
-    if (!group) return;
     for (auto& sec : sections) {
       if (sec.kind != Section::Compute) continue;
+      // launch units: ops, with a chain as its head
       std::vector<int> idx;
       for (int i = 0; i < (int)sec.ops.size(); ++i)
-        if (!sec.ops[i].fused) idx.push_back(i);
+        if (!sec.ops[i].fused && (sec.ops[i].chain < 0 || sec.ops[i].chain_head)) idx.push_back(i);
+      if (!group) {
+        for (int i : idx) {
+          sec.launches.push_back({i});
+          sec.launch_cls.push_back(sec.ops[i].chain >= 0 ? chain_cls(sec, sec.ops[i].chain) : sec.ops[i].cls);
+        }
+        continue;
+      }
       const int n = (int)idx.size();
       std::vector<std::vector<std::pair<uintptr_t, uintptr_t>>> rd(n), wr(n);
       for (int q = 0; q < n; ++q) {
-        const Op& op = sec.ops[idx[q]];
-        rd[q].push_back(span(op.a));
-        if (op.gemm) rd[q].push_back(span(op.b));
-        if (op.fuse_a >= 0) rd[q].push_back(span(op.fuse_a));
-        if (!op.assign) rd[q].push_back(span(op.out));
-        wr[q].push_back(span(op.out));
+        const Op& h = sec.ops[idx[q]];
+        std::vector<int> mem{idx[q]};
+        if (h.chain >= 0) mem = sec.chains[h.chain];
+        for (int i : mem) {
+          const Op& op = sec.ops[i];
+          rd[q].push_back(span(op.a));
+          if (op.gemm) rd[q].push_back(span(op.b));
+          if (op.fuse_a >= 0) rd[q].push_back(span(op.fuse_a));
+        }
+        if (!h.assign) rd[q].push_back(span(h.out));
+        wr[q].push_back(span(h.out));
       }
       auto meet = [](const std::vector<std::pair<uintptr_t, uintptr_t>>& x,
                      const std::vector<std::pair<uintptr_t, uintptr_t>>& y) {
@@ -1110,25 +1253,25 @@ struct ccq_hip_ctx {
         std::map<int64_t, std::vector<int>> cand;
         for (int i : here) {
           const Op& op = sec.ops[i];
-          if (!op.gemm || op.plan.kind != 0 || op.K < kGroupMinK) continue;
+          if (!op.gemm || op.chain >= 0 || op.plan.kind != 0 || op.K < kGroupMinK) continue;
           const ccqk::GemmArgs g = group_plan(op);
           if (ccqk::gemm_tiles(g) >= kGroupTiles) continue;
           const int64_t key = ccqk::gemm_group_key(g);
           if (key >= 0) cand[key].push_back(i);
         }
-        // a launch lasts as long as its longest workgroup: members of one
-        // group have K within a factor 1.5 (runs of the K-sorted candidates)
+        // a launch lasts about as long as its longest workgroups: members of
+        // one group have K within a factor 2 (runs of the K-sorted
+        // candidates), longest K first, so that the workgroups dispatched
+        // last are the short ones
         std::vector<std::vector<int>> runs;
         for (auto& kv : cand) {
           auto v = kv.second;
           std::stable_sort(v.begin(), v.end(),
-                           [&](int x, int y) { return sec.ops[x].K < sec.ops[y].K; });
+                           [&](int x, int y) { return sec.ops[x].K > sec.ops[y].K; });
           for (size_t a = 0; a < v.size();) {
             size_t b = a + 1;
-            while (b < v.size() && 2 * sec.ops[v[b]].K <= 3 * sec.ops[v[a]].K) ++b;
-            std::vector<int> run(v.begin() + a, v.begin() + b);
-            std::sort(run.begin(), run.end());
-            runs.push_back(run);
+            while (b < v.size() && sec.ops[v[a]].K <= 2 * sec.ops[v[b]].K) ++b;
+            runs.emplace_back(v.begin() + a, v.begin() + b);
             a = b;
           }
         }
@@ -1143,9 +1286,17 @@ struct ccq_hip_ctx {
             const size_t lo = v.size() * c / nch, hi = v.size() * (c + 1) / nch;
             if (hi - lo < 2) continue;
             std::vector<int> members(v.begin() + lo, v.begin() + hi);
-            group_at[members[0]] = members;
+            group_at[*std::min_element(members.begin(), members.end())] = members;
             grouped.insert(members.begin(), members.end());
           }
+        }
+        if (dump_schedule) {
+          for (int i : here)
+            std::fprintf(stderr, "[schedule] %s level %d op %d %s%s tiles %lld key %lld K %lld\n",
+                         sec.name.c_str(), lv, i, sec.ops[i].cls.c_str(),
+                         sec.ops[i].chain >= 0 ? " (chain head)" : "",
+                         (long long)ccqk::gemm_tiles(group_plan(sec.ops[i])),
+                         (long long)ccqk::gemm_group_key(group_plan(sec.ops[i])), (long long)unit_k(sec, i));
         }
         // this level in op order, each group where its first member was
         for (int i : here) {
@@ -1155,18 +1306,27 @@ struct ccq_hip_ctx {
             sec.launch_cls.push_back(group_cls(sec, it->second));
           } else if (!grouped.count(i)) {
             sec.launches.push_back({i});
-            sec.launch_cls.push_back(sec.ops[i].cls);
+            sec.launch_cls.push_back(sec.ops[i].chain >= 0 ? chain_cls(sec, sec.ops[i].chain) : sec.ops[i].cls);
           }
         }
       }
     }
   }
 
+  // K of a launch unit (a chain: its concatenated K)
+  int64_t unit_k(const Section& sec, int i) const {
+    const Op& op = sec.ops[i];
+    if (op.chain < 0) return op.K;
+    int64_t k = 0;
+    for (int j : sec.chains[op.chain]) k += sec.ops[j].K;
+    return k;
+  }
+
   std::string group_cls(const Section& sec, const std::vector<int>& members) const {
     std::map<std::string, int> shapes;
     for (int i : members) {
       const Op& op = sec.ops[i];
-      shapes[std::to_string(op.M) + "x" + std::to_string(op.N) + "x" + std::to_string(op.K)]++;
+      shapes[std::to_string(op.M) + "x" + std::to_string(op.N) + "x" + std::to_string(unit_k(sec, i))]++;
     }
     std::string t = "gemm:";
     bool first = true;
@@ -1231,8 +1391,8 @@ struct ccq_hip_ctx {
       gs[k].bk = gp.bk;
       // tile-row group of the XCD-aware order: about square blocks of the
       // member's tiles that one XCD receives (an eighth of the launch)
-      const int64_t tiles = ccqk::gemm_tiles(gs[k]);
-      const int64_t tm = ((gs[k].trans ? op.N : op.M) + gs[k].bm - 1) / gs[k].bm;
+      const int64_t tiles = ccqk::gemm_tiles(gp);
+      const int64_t tm = ((gp.trans ? op.N : op.M) + gp.bm - 1) / gp.bm;
       const double per_xcd = std::max(1.0, std::min((double)tiles, (double)total / 8.0));
       gs[k].group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, std::lround(std::sqrt(per_xcd))));
     }
@@ -1410,6 +1570,13 @@ struct ccq_hip_ctx {
         } else {
           for (size_t k = 0; k < sec.launches.size(); ++k) {
             const auto& L = sec.launches[k];
+            if (L.size() == 1 && sec.ops[L[0]].chain >= 0) {
+              const int c = sec.ops[L[0]].chain;
+              double fl = 0, by = 0;
+              for (int i : sec.chains[c]) fl += sec.ops[i].flops, by += sec.ops[i].bytes;
+              timed(sec.launch_cls[k], fl, by, [&] { run_chain(sec, c); });
+              continue;
+            }
             if (L.size() == 1) {
               const Op& op = sec.ops[L[0]];
               timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
@@ -1923,6 +2090,8 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (const char* g = std::getenv("CCQ_HIP_FUSE")) ctx->fuse = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_GROUP")) ctx->group = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_PACK")) ctx->pack_pairs = std::string(g) != "0";
+    if (const char* g = std::getenv("CCQ_HIP_CHAIN")) ctx->chain = std::string(g) != "0";
+    if (const char* g = std::getenv("CCQ_HIP_SCHEDULE")) ctx->dump_schedule = std::string(g) == "1";
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->layout();

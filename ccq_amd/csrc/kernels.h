@@ -144,6 +144,21 @@ struct GemmGroup {
 // cannot join a group (register kernel, split-K)
 int64_t gemm_group_key(const GemmArgs& g);
 hipError_t launch_gemm_group(hipStream_t s, const GemmArgs* gs, int n);
+
+// A chain of GEMMs accumulating into one output with the same tile mapping,
+// run as one product over the concatenated K ranges (LDS-tiled kernel, BK 32,
+// split-K over the concatenated K).  Entry i contributes alpha_i A_i B_i
+// (its trans flag: run transposed to match the output orientation); the
+// output arguments (C, its tables, accumulate, the folded addend) and the
+// launch plan (tile, nsplit, kchunk over the concatenated K, group, ws) are
+// the first entry's.
+constexpr int kMaxChain = 8;
+struct GemmChain {
+  GemmArgs e[kMaxChain];
+  int64_t koff[kMaxChain + 1];   // first concatenated k of each entry
+  int n;
+};
+hipError_t launch_gemm_chain(hipStream_t s, const GemmArgs* es, int n);
 hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
                            const int* colA, double* C, const int* rowC, const int* colC,
                            double alpha, int accumulate);

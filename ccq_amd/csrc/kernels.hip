@@ -42,9 +42,9 @@ __device__ __forceinline__ void put_c(double* p, double v, int accumulate) {
   if (accumulate) *p += v; else *p = v;
 }
 
-__device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n, double s) {
+__device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n, double s, double alpha) {
   const int64_t r = g.rowC[m], c = g.colC[n];
-  double v = g.alpha * s;
+  double v = alpha * s;
   if (g.D) v += g.dcoef * g.D[(int64_t)g.rowD[m] + g.colD[n]];
   put_c(g.C + r + c, v, g.accumulate);
   if (g.rowC2) {
@@ -53,6 +53,10 @@ __device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n,
     if (g.colC2) put_c(g.C + r2 + g.colC2[n], v, g.accumulate);
   }
   if (g.colC2) put_c(g.C + r + g.colC2[n], -v, g.accumulate);
+}
+
+__device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n, double s) {
+  store_c(g, m, n, s, g.alpha);
 }
 
 // sum_{i < cnt} q[i * stride] with four interleaved partial sums (fixed order)
@@ -262,34 +266,12 @@ void gemm_tab(GemmArgs g) {
   gemm_tab_tile<T, BM, BN, BK, AKF, BKF, KS>(g, tx, ty, tz, smem, kts);
 }
 
-// The same tile kernel over the tiles of a group of GEMMs (full K each).  The
-// operand layouts (k-contiguous or not) may differ between members: the
-// workgroup branches once, uniformly, to the body of its member's layouts.
-template <typename T, int BM, int BN, int BK>
-__global__ __launch_bounds__(256, !(sizeof(T) == 8 && BM * BN >= 128 * 128) ? 2 : 1)
-void gemm_tab_group(GemmGroup gg) {
-  constexpr int b11 = TabLds<T, BM, BN, BK, true, true, 1>::bytes;
-  constexpr int b10 = TabLds<T, BM, BN, BK, true, false, 1>::bytes;
-  constexpr int b01 = TabLds<T, BM, BN, BK, false, true, 1>::bytes;
-  constexpr int b00 = TabLds<T, BM, BN, BK, false, false, 1>::bytes;
-  constexpr int m1 = b11 > b10 ? b11 : b10, m2 = b01 > b00 ? b01 : b00;
-  __shared__ __align__(16) unsigned char smem[m1 > m2 ? m1 : m2];
-  __shared__ int kts[2 * kKCache];
-  int tx, ty;
-  const int mem = group_tile<BM, BN>(gg, tx, ty);
-  const GemmArgs& g = gg.g[mem];
-  if (g.a_kfast) {
-    if (g.b_kfast) gemm_tab_tile<T, BM, BN, BK, true, true, 1>(g, tx, ty, 0, smem, kts);
-    else gemm_tab_tile<T, BM, BN, BK, true, false, 1>(g, tx, ty, 0, smem, kts);
-  } else {
-    if (g.b_kfast) gemm_tab_tile<T, BM, BN, BK, false, true, 1>(g, tx, ty, 0, smem, kts);
-    else gemm_tab_tile<T, BM, BN, BK, false, false, 1>(g, tx, ty, 0, smem, kts);
-  }
-}
-
+// The K-slab loop of the tile kernel: acc += A(m0.., kbeg..kend) B(kbeg..kend, n0..)
+// (the caller zeroes acc; group grp of KS takes slabs grp, grp + KS, ...).
 template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
-__device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty, int tz, unsigned char* smem,
-                                              int* kts) {
+__device__ __forceinline__ void tile_mma(const GemmArgs& g, int64_t m0, int64_t n0, int64_t kbeg, int64_t kend,
+                                         unsigned char* smem, int* kts,
+                                         typename Mma<T>::acc_t (&acc)[BM / 32][BN / 32]) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
   using TA = LdsTile<T, BM, AKF, BK>;
@@ -301,10 +283,6 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
   const int tid = threadIdx.x & 255;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int64_t m0 = (int64_t)tx * BM;
-  const int64_t n0 = (int64_t)ty * BN;
-  const int64_t kbeg = (int64_t)tz * g.kchunk;
-  const int64_t kend = (kbeg + g.kchunk < g.K) ? kbeg + g.kchunk : g.K;
   const int wm = (wave & 1) * WM;
   const int wn = (wave >> 1) * WN;
 
@@ -326,13 +304,6 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
     __syncthreads();
   }
 
-  typename MM::acc_t acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i)
-#pragma unroll
-    for (int j = 0; j < TN; ++j) acc[i][j] = typename MM::acc_t{0, 0, 0, 0};
-
-  // slabs of this tile's K range; group grp takes slabs grp, grp + KS, ...
   // (a slab past the end loads zeros, so both groups run the same barriers)
   const int64_t nslabs = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   const int64_t nsteps = (nslabs + KS - 1) / KS;
@@ -365,6 +336,31 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
         for (int j = 0; j < TN; ++j) acc[i][j] = MM::step(af[i], bf[j], acc[i][j]);
     }
   }
+}
+
+template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+__device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty, int tz, unsigned char* smem,
+                                              int* kts) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  using MM = Mma<T>;
+  const int grp = threadIdx.x >> 8;
+  const int tid = threadIdx.x & 255;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int64_t m0 = (int64_t)tx * BM;
+  const int64_t n0 = (int64_t)ty * BN;
+  const int64_t kbeg = (int64_t)tz * g.kchunk;
+  const int64_t kend = (kbeg + g.kchunk < g.K) ? kbeg + g.kchunk : g.K;
+  const int wm = (wave & 1) * WM;
+  const int wn = (wave >> 1) * WN;
+
+  typename MM::acc_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = typename MM::acc_t{0, 0, 0, 0};
+  tile_mma<T, BM, BN, BK, AKF, BKF, KS>(g, m0, n0, kbeg, kend, smem, kts, acc);
 
   if (KS > 1) {
     // group 1 hands its accumulators to group 0 (slab order: group 0's sum
@@ -403,6 +399,128 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
         if (g.nsplit == 1) store_c(g, gm, gn, x);
         else store_partial(g, tz, gm, gn, x);
       }
+  }
+}
+
+// Entries e0 <= e < e1 of a chain accumulating into one output tile (tx, ty):
+// the workgroup walks the concatenated K range [kbeg, kend) (kend < 0: all of
+// it) entry by entry, each with its own operands and layouts, adds alpha_e
+// times the entry's partial product to an FP64 sum, and stores once with the
+// first entry's output arguments (split z of a split-K launch: into the
+// workspace).
+template <typename T, int BM, int BN, int BK>
+__device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1, int tx, int ty, int tz,
+                                             int64_t kbeg, int64_t kend, unsigned char* smem, int* kts) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int TM = WM / 16, TN = WN / 16;
+  using MM = Mma<T>;
+  const GemmArgs& o = es[e0];
+  const int lane = threadIdx.x & 63;
+  const int wave = (threadIdx.x & 255) >> 6;
+  const int64_t m0 = (int64_t)tx * BM;
+  const int64_t n0 = (int64_t)ty * BN;
+  const int wm = (wave & 1) * WM;
+  const int wn = (wave >> 1) * WN;
+
+  double sum[TM][TN][4];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sum[i][j][v] = 0.0;
+  int64_t off = 0;
+  for (int e = e0; e < e1; ++e) {
+    const GemmArgs& g = es[e];
+    const int64_t lo = kend < 0 ? off : (kbeg > off ? kbeg : off);
+    const int64_t hi = kend < 0 ? off + g.K : (kend < off + g.K ? kend : off + g.K);
+    const int64_t a = lo - off, b = hi - off;
+    off += g.K;
+    if (a >= b) continue;
+    typename MM::acc_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = typename MM::acc_t{0, 0, 0, 0};
+    // the previous entry's waves are past their last LDS reads
+    if (e > e0) __syncthreads();
+    if (g.a_kfast) {
+      if (g.b_kfast) tile_mma<T, BM, BN, BK, true, true, 1>(g, m0, n0, a, b, smem, kts, acc);
+      else tile_mma<T, BM, BN, BK, true, false, 1>(g, m0, n0, a, b, smem, kts, acc);
+    } else {
+      if (g.b_kfast) tile_mma<T, BM, BN, BK, false, true, 1>(g, m0, n0, a, b, smem, kts, acc);
+      else tile_mma<T, BM, BN, BK, false, false, 1>(g, m0, n0, a, b, smem, kts, acc);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sum[i][j][v] += g.alpha * (double)acc[i][j][v];
+  }
+
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gn = n0 + wn + 16 * j + (lane & 15);
+    if (gn >= o.N) continue;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t gm = m0 + wm + 16 * i + MM::row(lane, v);
+        if (gm >= o.M) continue;
+        if (kend < 0 || o.nsplit == 1) store_c(o, gm, gn, sum[i][j][v], 1.0);
+        else store_partial(o, tz, gm, gn, sum[i][j][v]);
+      }
+  }
+}
+
+// LDS of the kernels that switch between operand layouts per entry
+template <typename T, int BM, int BN, int BK>
+struct AnyLds {
+  static constexpr int b11 = TabLds<T, BM, BN, BK, true, true, 1>::bytes;
+  static constexpr int b10 = TabLds<T, BM, BN, BK, true, false, 1>::bytes;
+  static constexpr int b01 = TabLds<T, BM, BN, BK, false, true, 1>::bytes;
+  static constexpr int b00 = TabLds<T, BM, BN, BK, false, false, 1>::bytes;
+  static constexpr int m1 = b11 > b10 ? b11 : b10, m2 = b01 > b00 ? b01 : b00;
+  static constexpr int bytes = m1 > m2 ? m1 : m2;
+};
+
+// A chain of GEMMs accumulating into one output with the same tile mapping
+// (C (+)= sum_e alpha_e A_e B_e (+ dcoef D)), as one product over the
+// concatenated K ranges, split along the concatenated K (split z of tile
+// (tx, ty) covers [z kchunk, (z + 1) kchunk)).
+template <typename T, int BM, int BN, int BK>
+__global__ __launch_bounds__(256, !(sizeof(T) == 8 && BM * BN >= 128 * 128) ? 2 : 1)
+void gemm_tab_chain(GemmChain ch) {
+  __shared__ __align__(16) unsigned char smem[AnyLds<T, BM, BN, BK>::bytes];
+  __shared__ int kts[2 * kKCache];
+  int tx, ty, tz;
+  tile_coords(ch.e[0].group, tx, ty, tz);
+  const int64_t ktot = ch.koff[ch.n];
+  const int64_t kbeg = (int64_t)tz * ch.e[0].kchunk;
+  const int64_t kend = (kbeg + ch.e[0].kchunk < ktot) ? kbeg + ch.e[0].kchunk : ktot;
+  tile_entries<T, BM, BN, BK>(ch.e, 0, ch.n, tx, ty, tz, kbeg, kend, smem, kts);
+}
+
+// The same tile kernel over the tiles of a group of independent GEMMs (full K
+// each).  The operand layouts (k-contiguous or not) may differ between
+// members: the workgroup branches once, uniformly, to the body of its
+// member's layouts.
+template <typename T, int BM, int BN, int BK>
+__global__ __launch_bounds__(256, !(sizeof(T) == 8 && BM * BN >= 128 * 128) ? 2 : 1)
+void gemm_tab_group(GemmGroup gg) {
+  __shared__ __align__(16) unsigned char smem[AnyLds<T, BM, BN, BK>::bytes];
+  __shared__ int kts[2 * kKCache];
+  int tx, ty;
+  const int mem = group_tile<BM, BN>(gg, tx, ty);
+  const GemmArgs& g = gg.g[mem];
+  if (g.a_kfast) {
+    if (g.b_kfast) gemm_tab_tile<T, BM, BN, BK, true, true, 1>(g, tx, ty, 0, smem, kts);
+    else gemm_tab_tile<T, BM, BN, BK, true, false, 1>(g, tx, ty, 0, smem, kts);
+  } else {
+    if (g.b_kfast) gemm_tab_tile<T, BM, BN, BK, false, true, 1>(g, tx, ty, 0, smem, kts);
+    else gemm_tab_tile<T, BM, BN, BK, false, false, 1>(g, tx, ty, 0, smem, kts);
   }
 }
 
@@ -1281,12 +1399,7 @@ static hipError_t launch_kernels(hipStream_t s, const GemmArgs& g) {
   return hipGetLastError();
 }
 
-hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
-  const GemmArgs g = g0.trans ? transposed(g0) : g0;
-  if (g.M <= 0 || g.N <= 0) return hipSuccess;
-  if (g.nsplit > 1 && !g.ws) return hipErrorInvalidValue;
-  hipError_t e = launch_kernels(s, g);
-  if (e != hipSuccess || g.nsplit == 1) return e;
+static hipError_t launch_reduce(hipStream_t s, const GemmArgs& g) {
   // threads per element: enough to cover nsplit in a few loads each
   int pshift = 0;
   while ((1 << pshift) < g.nsplit && (1 << pshift) < 32 && g.M * g.N * (2 << pshift) <= 262144)
@@ -1296,6 +1409,56 @@ hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
                      dim3((unsigned)cdiv(g.N, per_block), (unsigned)std::min<int64_t>(g.M, 65535)),
                      dim3(256), 0, s, g, pshift);
   return hipGetLastError();
+}
+
+hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
+  const GemmArgs g = g0.trans ? transposed(g0) : g0;
+  if (g.M <= 0 || g.N <= 0) return hipSuccess;
+  if (g.nsplit > 1 && !g.ws) return hipErrorInvalidValue;
+  hipError_t e = launch_kernels(s, g);
+  if (e != hipSuccess || g.nsplit == 1) return e;
+  return launch_reduce(s, g);
+}
+
+template <int BM, int BN>
+static void launch_chain_k(hipStream_t s, const GemmChain& ch) {
+  const GemmArgs& o = ch.e[0];
+  const dim3 grid((unsigned)cdiv(o.M, BM), (unsigned)cdiv(o.N, BN), (unsigned)o.nsplit), blk(256);
+  if (o.prec == 1) hipLaunchKernelGGL((gemm_tab_chain<float, BM, BN, 32>), grid, blk, 0, s, ch);
+  else hipLaunchKernelGGL((gemm_tab_chain<double, BM, BN, 32>), grid, blk, 0, s, ch);
+}
+
+hipError_t launch_gemm_chain(hipStream_t s, const GemmArgs* es, int n) {
+  if (n < 1 || n > kMaxChain) return hipErrorInvalidValue;
+  GemmChain ch{};
+  ch.n = n;
+  ch.koff[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    ch.e[i] = es[i].trans ? transposed(es[i]) : es[i];
+    ch.koff[i + 1] = ch.koff[i] + ch.e[i].K;
+  }
+  const GemmArgs& o = ch.e[0];
+  if (o.M <= 0 || o.N <= 0) return hipSuccess;
+  if (o.kind != 0 || o.bk != 32 || o.ksplit != 1 || o.nsplit < 1 || (o.nsplit > 1 && !o.ws))
+    return hipErrorInvalidValue;
+  for (int i = 1; i < n; ++i)
+    if (ch.e[i].M != o.M || ch.e[i].N != o.N || ch.e[i].prec != o.prec) return hipErrorInvalidValue;
+  switch (o.bm * 1000 + o.bn) {
+    case 128128: launch_chain_k<128, 128>(s, ch); break;
+    case 128064: launch_chain_k<128, 64>(s, ch); break;
+    case 64128: launch_chain_k<64, 128>(s, ch); break;
+    case 64064: launch_chain_k<64, 64>(s, ch); break;
+    case 64032: launch_chain_k<64, 32>(s, ch); break;
+    case 32064: launch_chain_k<32, 64>(s, ch); break;
+    case 32032: launch_chain_k<32, 32>(s, ch); break;
+    default: return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || o.nsplit == 1) return e;
+  // the reduction stores the summed, already scaled, partial products
+  GemmArgs r = o;
+  r.alpha = 1.0;
+  return launch_reduce(s, r);
 }
 
 // Grouped launches run the KS = 1, BK = 32 tile kernel; members share the

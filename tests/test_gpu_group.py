@@ -47,7 +47,7 @@ def _engine(c, method, monkeypatch, group, graph="0"):
 GROUPED = re.compile(r"^gemm:.*\*.*g$")
 
 
-@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 64, 8, 8), ("CCSD", 64, 8, 7), ("CCSDT", 30, 6, 6)])
+@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 64, 8, 8), ("CCSD", 72, 9, 8), ("CCSDT", 30, 6, 6)])
 def test_grouped_launches_match_single(method, n, o, ob, monkeypatch):
     _say(f"case grouped launches {method} n={n} o={o} ob={ob}")
     c = Case(n, o, 2 * n, 23, occ_b=ob)
@@ -140,7 +140,7 @@ def test_packed_pairs_match_full(method, n, o, ob, monkeypatch):
         eng.update()
         rep = eng.profile_report()
         eng.profile(False)
-        packed = [k for k in rep if k.startswith("gemm:") and (k.endswith("p") or k.endswith("pg"))]
+        packed = [k for k in rep if k.startswith("gemm:") and "p" in k.rsplit("/", 1)[1]]
         assert bool(packed) == (pack == "1"), sorted(rep)[:12]
         es = [eng.energy()]
         for _ in range(2):
@@ -153,3 +153,47 @@ def test_packed_pairs_match_full(method, n, o, ob, monkeypatch):
     assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
     if q0 is not None:
         assert np.abs(q1 - q0).max() <= 1e-11 * max(np.abs(q0).max(), 1e-12)
+
+
+@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 64, 8, 8), ("CCSD", 64, 8, 7)])
+def test_chained_launches_match_single(method, n, o, ob, monkeypatch):
+    """GEMM chains (consecutive products into one output with one tile
+    mapping, run over their concatenated K) against one launch per GEMM
+    (CCQ_HIP_CHAIN=0): three sweeps agree to 1e-11, graph replays of the
+    chained sweep equal its eager launches."""
+    _say(f"case chained launches {method} n={n} o={o} ob={ob}")
+    c = Case(n, o, 2 * n, 41, occ_b=ob)
+    vec = _state(c, method, 7)
+    out = []
+    for chain in ("0", "1"):
+        eng = _engine_env(c, method, monkeypatch, {"CCQ_HIP_CHAIN": chain, "CCQ_HIP_GRAPH": "0"}, shift=0.02)
+        eng.set_t(vec)
+        eng.profile(True)
+        eng.update()
+        rep = eng.profile_report()
+        eng.profile(False)
+        chained = [k for k in rep if k.startswith("gemm:") and "c" in k.rsplit("/", 1)[1]]
+        if chain == "1":
+            _say("  chained classes: " + ", ".join(chained[:6]))
+        assert bool(chained) == (chain == "1"), sorted(rep)[:12]
+        es = [eng.energy()]
+        for _ in range(2):
+            eng.update()
+            es.append(eng.energy())
+        out.append((np.array(es), eng.get_t()))
+        eng.close()
+    (e0, t0), (e1, t1) = out
+    np.testing.assert_allclose(e1, e0, rtol=1e-11, atol=0)
+    assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
+    engines = [_engine_env(c, method, monkeypatch, {"CCQ_HIP_GRAPH": g}, shift=0.02) for g in ("1", "0")]
+    for eng in engines:
+        eng.set_t(vec)
+    for rep in range(3):
+        res = []
+        for eng in engines:
+            eng.update()
+            res.append((eng.energy(), eng.get_t()))
+        assert res[0][0] == res[1][0]
+        np.testing.assert_array_equal(res[0][1], res[1][1])
+    for eng in engines:
+        eng.close()
