@@ -933,8 +933,10 @@ struct ccq_hip_ctx {
                       : (!ns.empty() && stride_of(B, op.lb, ns.back()) == 1 ? 0 : 1));
           g.kA_aff = affine_stride(kA);
           g.kB_aff = affine_stride(kB);
-          g.c_mfast = affine_stride(rC) == 1 ? 1 : 0;
-          g.c_nfast = affine_stride(cC) == 1 ? 1 : 0;
+          // C contiguous along m (n): its fastest m (n) label has stride 1,
+          // so consecutive rows (columns) are adjacent in runs of that extent
+          g.c_mfast = !ms.empty() && stride_of(C, op.lc, ms.back()) == 1 ? 1 : 0;
+          g.c_nfast = !ns.empty() && stride_of(C, op.lc, ns.back()) == 1 ? 1 : 0;
           ccqk::gemm_plan(g, kWsCap);
           ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(g));
           op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +
