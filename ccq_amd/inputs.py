@@ -67,6 +67,20 @@ def _logical(val: str) -> bool:
     raise ValueError(f"not a logical: {val!r}")
 
 
+# ACC diagram scale factors (src/parser.f90:529-543; REAL*4 lists, default
+# 1.0 at src/parser.f90:838-842) and their lengths
+ACC_KEYS = {"t2t2_t2": 5, "t3_t2": 2, "t1t3_t2": 4, "t2t2_t3": 3, "t2t3_t3": 5}
+
+
+def _fortran_reals(val: str):
+    """Values of a Fortran list-directed real list ('1.0, 1.0', '5*1.0', 1d0)."""
+    out = []
+    for tok in val.replace(",", " ").split():
+        rep, _, x = tok.rpartition("*")
+        out += [float(x.replace("d", "e").replace("D", "E"))] * (int(rep) if rep else 1)
+    return out
+
+
 def parse_config(path: str) -> Config:
     cfg = Config()
     base = os.path.dirname(os.path.abspath(path))
@@ -125,6 +139,12 @@ def parse_config(path: str) -> Config:
             cfg.label = val
         elif key == "keep_bin":
             cfg.keep_bin = _logical(val)
+        elif key in ACC_KEYS:
+            # the engine computes plain CC: ACC scaling only at its default
+            vals = _fortran_reals(val)
+            if len(vals) < ACC_KEYS[key] or any(v != 1.0 for v in vals[:ACC_KEYS[key]]):
+                raise ValueError(f"ACC scale factors {key} = {val} are not supported "
+                                 "(only the default 1.0)")
         # any other keyword is ignored, as in ccq
     if cfg.mult == 0:
         cfg.mult = 1

@@ -24,8 +24,9 @@ program ccq_drv
     real(dp), allocatable :: e1int(:,:), e2int(:,:,:,:), f_a(:,:), f_b(:,:)
     real(dp), allocatable :: v_aa(:,:,:,:), v_ab(:,:,:,:)
     real(dp) :: en_repul, en_ref, e_cor, e_left, e23(4), t0
-    integer(c_int64_t) :: i, j, k, a, b, ios, unit_in, itol, eq
+    integer(c_int64_t) :: i, j, k, a, b, ios, unit_in, itol, eq, nacc
     logical :: rhf, converged, cct3
+    real :: accv(5)
     type(ccq_hip_dims) :: dims
     type(ccq_hip_opts) :: opts
     type(c_ptr) :: ctx
@@ -73,6 +74,23 @@ program ccq_drv
         case ('calc_type'); calc_type = val
         case ('act_occ'); read(val, *) act_occ
         case ('act_unocc'); read(val, *) act_unocc
+        case ('t2t2_t2', 't3_t2', 't1t3_t2', 't2t2_t3', 't2t3_t3')
+            ! ACC diagram scale factors (src/parser.f90:529-543, read as
+            ! REAL*4 lists of 5, 2, 4, 3, 5 values): this engine computes
+            ! plain CC, the default 1.0 (src/parser.f90:838-842), only
+            select case (trim(key))
+            case ('t3_t2'); nacc = 2
+            case ('t1t3_t2'); nacc = 4
+            case ('t2t2_t3'); nacc = 3
+            case default; nacc = 5
+            end select
+            accv = 1.0
+            read(val, *, iostat=ios) accv(1:nacc)
+            if (ios /= 0 .or. any(accv(1:nacc) /= 1.0)) then
+                write(0, '(a)') 'ccq_drv: ACC scale factors '//trim(key)//' = '//trim(val)// &
+                    ' are not supported (only the default 1.0)'
+                call exit(1)
+            end if
         end select
     end do
     close(unit_in)
