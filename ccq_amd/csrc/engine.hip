@@ -1098,10 +1098,13 @@ struct ccq_hip_ctx {
         for (; q < live.size() && mem.size() < (size_t)ccqk::kMaxChain; ++q) {
           const Op& x = ops[live[q]];
           if (!eligible(live[q]) || x.out != h.out || x.assign || x.fuse_a >= 0) break;
+          // same tile mapping and, in that orientation, the same operand layouts
           int f = -1;
-          if (x.M == h.M && x.N == h.N && same(x.t_rowC, h.t_rowC, h.M) && same(x.t_colC, h.t_colC, h.N))
+          if (x.M == h.M && x.N == h.N && same(x.t_rowC, h.t_rowC, h.M) && same(x.t_colC, h.t_colC, h.N) &&
+              x.plan.a_kfast == h.plan.a_kfast && x.plan.b_kfast == h.plan.b_kfast)
             f = 0;
-          else if (x.M == h.N && x.N == h.M && same(x.t_rowC, h.t_colC, h.N) && same(x.t_colC, h.t_rowC, h.M))
+          else if (x.M == h.N && x.N == h.M && same(x.t_rowC, h.t_colC, h.N) && same(x.t_colC, h.t_rowC, h.M) &&
+                   x.plan.b_kfast == h.plan.a_kfast && x.plan.a_kfast == h.plan.b_kfast)
             f = 1;
           if (f < 0) break;
           mem.push_back(live[q]);
@@ -1116,7 +1119,8 @@ struct ccq_hip_ctx {
         cp.K = ktot;
         cp.kA_aff = cp.kB_aff = -1;
         ccqk::gemm_plan(cp, kWsCap);
-        if (cp.kind != 0) {
+        // (the chain kernel spills registers with tiles above 64 x 64)
+        if (cp.kind != 0 || cp.bm * cp.bn > 64 * 64) {
           ++p;
           continue;
         }

@@ -145,7 +145,8 @@ struct GemmGroup {
 int64_t gemm_group_key(const GemmArgs& g);
 hipError_t launch_gemm_group(hipStream_t s, const GemmArgs* gs, int n);
 
-// A chain of GEMMs accumulating into one output with the same tile mapping,
+// A chain of GEMMs accumulating into one output with the same tile mapping
+// and the same operand layouts (k-contiguous or not, after transposition),
 // run as one product over the concatenated K ranges (LDS-tiled kernel, BK 32,
 // split-K over the concatenated K).  Entry i contributes alpha_i A_i B_i
 // (its trans flag: run transposed to match the output orientation); the

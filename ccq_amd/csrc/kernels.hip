@@ -239,6 +239,82 @@ struct Mma<float> {
 // combine their accumulators through LDS at the end in a fixed order -- the
 // K split of a global split-K without its partial products in HBM and its
 // reduction launch.
+// Epilogue of one tile: C[rowC[m] + colC[n]] (+)= alpha val (+ dcoef D[..]).
+// Each lane loads its row offsets once and, per column, all its old C and
+// addend values in one batch before storing: the compiler has to assume a
+// store through C may change the offset tables, so an element-by-element
+// epilogue serialises a table load, a C load and the store for every
+// element.  Outputs with mirrored images (packed pairs) take the
+// element-by-element path.
+template <int TM, int TN, typename MM, typename V>
+__device__ __forceinline__ void store_tile(const GemmArgs& g, int64_t m0, int64_t n0, int wm, int wn, int lane,
+                                           double alpha, const V (&val)[TM][TN]) {
+  if (g.rowC2 || g.colC2) {
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+      const int64_t gn = n0 + wn + 16 * j + (lane & 15);
+      if (gn >= g.N) continue;
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t gm = m0 + wm + 16 * i + MM::row(lane, v);
+          if (gm < g.M) store_c(g, gm, gn, (double)val[i][j][v], alpha);
+        }
+    }
+    return;
+  }
+  int rc[TM][4], rd[TM][4];   // int32 offsets as in the tables; -1: row out of range
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t gm = m0 + wm + 16 * i + MM::row(lane, v);
+      const bool in = gm < g.M;
+      rc[i][v] = in ? g.rowC[gm] : -1;
+      rd[i][v] = (in && g.D) ? g.rowD[gm] : 0;
+    }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int64_t gn = n0 + wn + 16 * j + (lane & 15);
+    if (gn >= g.N) continue;
+    const int64_t cc = g.colC[gn];
+    double out[TM][4];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[i][v] = alpha * (double)val[i][j][v];
+    if (g.D) {
+      const int64_t cd = g.colD[gn];
+      double dv[TM][4];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dv[i][v] = rc[i][v] >= 0 ? g.D[(int64_t)rd[i][v] + cd] : 0.0;
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[i][v] += g.dcoef * dv[i][v];
+    }
+    if (g.accumulate) {
+      double old[TM][4];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) old[i][v] = rc[i][v] >= 0 ? g.C[(int64_t)rc[i][v] + cc] : 0.0;
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[i][v] = old[i][v] + out[i][v];
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (rc[i][v] >= 0) g.C[(int64_t)rc[i][v] + cc] = out[i][v];
+  }
+}
+
 // LDS bytes of one tile-kernel variant
 template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
 struct TabLds {
@@ -385,6 +461,10 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
         for (int v = 0; v < 4; ++v) acc[i][j][v] += red[(((i * TN + j) * 4 + v) << 8) + tid];
   }
 
+  if (g.nsplit == 1) {
+    store_tile<TM, TN, MM>(g, m0, n0, wm, wn, lane, g.alpha, acc);
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     const int64_t gn = n0 + wn + 16 * j + (lane & 15);
@@ -395,9 +475,7 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
       for (int v = 0; v < 4; ++v) {
         const int64_t gm = m0 + wm + 16 * i + MM::row(lane, v);
         if (gm >= g.M) continue;
-        const double x = (double)acc[i][j][v];
-        if (g.nsplit == 1) store_c(g, gm, gn, x);
-        else store_partial(g, tz, gm, gn, x);
+        store_partial(g, tz, gm, gn, (double)acc[i][j][v]);
       }
   }
 }
@@ -408,7 +486,7 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
 // times the entry's partial product to an FP64 sum, and stores once with the
 // first entry's output arguments (split z of a split-K launch: into the
 // workspace).
-template <typename T, int BM, int BN, int BK>
+template <typename T, int BM, int BN, int BK, bool AKF, bool BKF>
 __device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1, int tx, int ty, int tz,
                                              int64_t kbeg, int64_t kend, unsigned char* smem, int* kts) {
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -422,13 +500,11 @@ __device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1,
   const int wm = (wave & 1) * WM;
   const int wn = (wave >> 1) * WN;
 
-  double sum[TM][TN][4];
+  d4 sum[TM][TN];
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) sum[i][j][v] = 0.0;
+    for (int j = 0; j < TN; ++j) sum[i][j] = d4{0.0, 0.0, 0.0, 0.0};
   int64_t off = 0;
   for (int e = e0; e < e1; ++e) {
     const GemmArgs& g = es[e];
@@ -444,13 +520,7 @@ __device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1,
       for (int j = 0; j < TN; ++j) acc[i][j] = typename MM::acc_t{0, 0, 0, 0};
     // the previous entry's waves are past their last LDS reads
     if (e > e0) __syncthreads();
-    if (g.a_kfast) {
-      if (g.b_kfast) tile_mma<T, BM, BN, BK, true, true, 1>(g, m0, n0, a, b, smem, kts, acc);
-      else tile_mma<T, BM, BN, BK, true, false, 1>(g, m0, n0, a, b, smem, kts, acc);
-    } else {
-      if (g.b_kfast) tile_mma<T, BM, BN, BK, false, true, 1>(g, m0, n0, a, b, smem, kts, acc);
-      else tile_mma<T, BM, BN, BK, false, false, 1>(g, m0, n0, a, b, smem, kts, acc);
-    }
+    tile_mma<T, BM, BN, BK, AKF, BKF, 1>(g, m0, n0, a, b, smem, kts, acc);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -459,6 +529,10 @@ __device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1,
         for (int v = 0; v < 4; ++v) sum[i][j][v] += g.alpha * (double)acc[i][j][v];
   }
 
+  if (kend < 0 || o.nsplit == 1) {
+    store_tile<TM, TN, MM>(o, m0, n0, wm, wn, lane, 1.0, sum);
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < TN; ++j) {
     const int64_t gn = n0 + wn + 16 * j + (lane & 15);
@@ -469,8 +543,7 @@ __device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1,
       for (int v = 0; v < 4; ++v) {
         const int64_t gm = m0 + wm + 16 * i + MM::row(lane, v);
         if (gm >= o.M) continue;
-        if (kend < 0 || o.nsplit == 1) store_c(o, gm, gn, sum[i][j][v], 1.0);
-        else store_partial(o, tz, gm, gn, sum[i][j][v]);
+        store_partial(o, tz, gm, gn, sum[i][j][v]);
       }
   }
 }
@@ -487,20 +560,20 @@ struct AnyLds {
 };
 
 // A chain of GEMMs accumulating into one output with the same tile mapping
-// (C (+)= sum_e alpha_e A_e B_e (+ dcoef D)), as one product over the
-// concatenated K ranges, split along the concatenated K (split z of tile
-// (tx, ty) covers [z kchunk, (z + 1) kchunk)).
-template <typename T, int BM, int BN, int BK>
+// and operand layouts (C (+)= sum_e alpha_e A_e B_e (+ dcoef D)), as one
+// product over the concatenated K ranges, split along the concatenated K
+// (split z of tile (tx, ty) covers [z kchunk, (z + 1) kchunk)).
+template <typename T, int BM, int BN, int BK, bool AKF, bool BKF>
 __global__ __launch_bounds__(256, !(sizeof(T) == 8 && BM * BN >= 128 * 128) ? 2 : 1)
 void gemm_tab_chain(GemmChain ch) {
-  __shared__ __align__(16) unsigned char smem[AnyLds<T, BM, BN, BK>::bytes];
+  __shared__ __align__(16) unsigned char smem[TabLds<T, BM, BN, BK, AKF, BKF, 1>::bytes];
   __shared__ int kts[2 * kKCache];
   int tx, ty, tz;
   tile_coords(ch.e[0].group, tx, ty, tz);
   const int64_t ktot = ch.koff[ch.n];
   const int64_t kbeg = (int64_t)tz * ch.e[0].kchunk;
   const int64_t kend = (kbeg + ch.e[0].kchunk < ktot) ? kbeg + ch.e[0].kchunk : ktot;
-  tile_entries<T, BM, BN, BK>(ch.e, 0, ch.n, tx, ty, tz, kbeg, kend, smem, kts);
+  tile_entries<T, BM, BN, BK, AKF, BKF>(ch.e, 0, ch.n, tx, ty, tz, kbeg, kend, smem, kts);
 }
 
 // The same tile kernel over the tiles of a group of independent GEMMs (full K
@@ -640,6 +713,12 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
   }
 
   if (wk != 0) return;
+  // lane l, register v of 16x16 block (i, j) holds row 16i + l/16 + 4v and
+  // column 16j + l%16: the FP64 MFMA output layout of the tile epilogue
+  if (g.nsplit == 1) {
+    store_tile<RM, RN, Mma<double>>(g, wm0, n0, 0, 0, lane, g.alpha, acc);
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < RN; ++j) {
     const int64_t gn = n0 + 16 * j + r;
@@ -650,8 +729,7 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
       for (int v = 0; v < 4; ++v) {
         const int64_t gm = wm0 + 16 * i + q + 4 * v;
         if (gm >= g.M) continue;
-        if (g.nsplit == 1) store_c(g, gm, gn, acc[i][j][v]);
-        else store_partial(g, tz, gm, gn, acc[i][j][v]);
+        store_partial(g, tz, gm, gn, acc[i][j][v]);
       }
   }
 }
@@ -1420,12 +1498,21 @@ hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
   return launch_reduce(s, g);
 }
 
+template <typename T, int BM, int BN>
+static void launch_chain_t(hipStream_t s, dim3 grid, const GemmChain& ch) {
+  const GemmArgs& o = ch.e[0];
+  if (o.a_kfast && o.b_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, BM, BN, 32, true, true>), grid, dim3(256), 0, s, ch);
+  else if (o.a_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, BM, BN, 32, true, false>), grid, dim3(256), 0, s, ch);
+  else if (o.b_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, BM, BN, 32, false, true>), grid, dim3(256), 0, s, ch);
+  else hipLaunchKernelGGL((gemm_tab_chain<T, BM, BN, 32, false, false>), grid, dim3(256), 0, s, ch);
+}
+
 template <int BM, int BN>
 static void launch_chain_k(hipStream_t s, const GemmChain& ch) {
   const GemmArgs& o = ch.e[0];
-  const dim3 grid((unsigned)cdiv(o.M, BM), (unsigned)cdiv(o.N, BN), (unsigned)o.nsplit), blk(256);
-  if (o.prec == 1) hipLaunchKernelGGL((gemm_tab_chain<float, BM, BN, 32>), grid, blk, 0, s, ch);
-  else hipLaunchKernelGGL((gemm_tab_chain<double, BM, BN, 32>), grid, blk, 0, s, ch);
+  const dim3 grid((unsigned)cdiv(o.M, BM), (unsigned)cdiv(o.N, BN), (unsigned)o.nsplit);
+  if (o.prec == 1) launch_chain_t<float, BM, BN>(s, grid, ch);
+  else launch_chain_t<double, BM, BN>(s, grid, ch);
 }
 
 hipError_t launch_gemm_chain(hipStream_t s, const GemmArgs* es, int n) {
@@ -1442,7 +1529,9 @@ hipError_t launch_gemm_chain(hipStream_t s, const GemmArgs* es, int n) {
   if (o.kind != 0 || o.bk != 32 || o.ksplit != 1 || o.nsplit < 1 || (o.nsplit > 1 && !o.ws))
     return hipErrorInvalidValue;
   for (int i = 1; i < n; ++i)
-    if (ch.e[i].M != o.M || ch.e[i].N != o.N || ch.e[i].prec != o.prec) return hipErrorInvalidValue;
+    if (ch.e[i].M != o.M || ch.e[i].N != o.N || ch.e[i].prec != o.prec ||
+        ch.e[i].a_kfast != o.a_kfast || ch.e[i].b_kfast != o.b_kfast)
+      return hipErrorInvalidValue;
   switch (o.bm * 1000 + o.bn) {
     case 128128: launch_chain_k<128, 128>(s, ch); break;
     case 128064: launch_chain_k<128, 64>(s, ch); break;
