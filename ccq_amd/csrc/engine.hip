@@ -666,10 +666,11 @@ struct ccq_hip_ctx {
         std::string key = std::to_string((uintptr_t)basep);
         for (int i = 0; i < t.rank; ++i) key += t.spec[i];
         auto hit = view_key.find(key);
+        const std::vector<int> order = view_order(t);
         int64_t d = 1;
         for (int i = t.rank - 1; i >= 0; --i) {
-          t.strides[i] = d;
-          d *= t.dims[i];
+          t.strides[order[i]] = d;
+          d *= t.dims[order[i]];
         }
         if (hit != view_key.end()) {
           t.ptr = hit->second;
@@ -687,6 +688,54 @@ struct ccq_hip_ctx {
 
   std::map<std::string, double*> view_key;
   std::vector<int> view_owner;
+
+  // Storage order of a view's dense copy (outermost index first).  A view
+  // whose every GEMM consumer contracts it with a small operand (at most
+  // 1024 elements: the T1-dressing products) keeps the indices summed by its
+  // first such consumer outermost, so that the rows those products read are
+  // contiguous; any other view keeps its index order.  A function of the
+  // view's base and spec, so identical views sharing a copy agree.
+  // CCQ_HIP_VIEWPERM=0 keeps every view in index order.
+  bool view_perm = true;
+
+  std::vector<int> view_order(const Tensor& t) const {
+    std::vector<int> ident(t.rank);
+    for (int i = 0; i < t.rank; ++i) ident[i] = i;
+    if (!view_perm) return ident;
+    auto same_view = [&](int x) {
+      const Tensor& u = tensors[x];
+      if (u.kind != TKind::View || u.base != t.base || u.rank != t.rank) return false;
+      for (int i = 0; i < t.rank; ++i)
+        if (u.spec[i] != t.spec[i]) return false;
+      return true;
+    };
+    std::string labels, lc;
+    bool any = false;
+    for (const auto& sec : sections)
+      for (const auto& op : sec.ops) {
+        if (!op.gemm) continue;
+        const bool va = same_view(op.a), vb = same_view(op.b);
+        if (!va && !vb) continue;
+        if (va && vb) return ident;
+        const Tensor& other = tensors[va ? op.b : op.a];
+        if (other.kind == TKind::View) return ident;
+        int64_t n = 1;
+        for (int i = 0; i < other.rank; ++i) n *= other.dims[i];
+        if (n > 1024) return ident;
+        if (!any) {
+          labels = va ? op.la : op.lb;
+          lc = op.lc;
+          any = true;
+        }
+      }
+    if (!any) return ident;
+    std::vector<int> order;
+    for (int i = 0; i < t.rank; ++i)
+      if (lc.find(labels[i]) == std::string::npos) order.push_back(i);
+    for (int i = 0; i < t.rank; ++i)
+      if (lc.find(labels[i]) != std::string::npos) order.push_back(i);
+    return order;
+  }
 
   // tables that copy each distinct view out of its base array
   void plan_gathers() {
@@ -2097,6 +2146,7 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (const char* g = std::getenv("CCQ_HIP_GROUP")) ctx->group = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_PACK")) ctx->pack_pairs = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_CHAIN")) ctx->chain = std::string(g) != "0";
+    if (const char* g = std::getenv("CCQ_HIP_VIEWPERM")) ctx->view_perm = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_SCHEDULE")) ctx->dump_schedule = std::string(g) == "1";
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
