@@ -197,3 +197,27 @@ def test_chained_launches_match_single(method, n, o, ob, monkeypatch):
         np.testing.assert_array_equal(res[0][1], res[1][1])
     for eng in engines:
         eng.close()
+
+
+@pytest.mark.parametrize("switch", ["CCQ_HIP_FUSE", "CCQ_HIP_VIEWPERM"])
+@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 24, 5, 4), ("CCSDT", 12, 3, 2)])
+def test_switch_off_matches_default(switch, method, n, o, ob, monkeypatch):
+    """Add folding and the consumer-ordered integral views change only the
+    launch structure and memory order: with the switch off, three sweeps
+    agree with the default to 1e-11."""
+    _say(f"case switch {switch} {method} n={n} o={o} ob={ob}")
+    c = Case(n, o, 2 * n, 43, occ_b=ob)
+    vec = _state(c, method, 13)
+    out = []
+    for val in ("0", "1"):
+        eng = _engine_env(c, method, monkeypatch, {switch: val, "CCQ_HIP_GRAPH": "0"}, shift=0.02)
+        eng.set_t(vec)
+        es = []
+        for _ in range(3):
+            eng.update()
+            es.append(eng.energy())
+        out.append((np.array(es), eng.get_t()))
+        eng.close()
+    (e0, t0), (e1, t1) = out
+    np.testing.assert_allclose(e1, e0, rtol=1e-11, atol=0)
+    assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
