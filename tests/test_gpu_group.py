@@ -221,3 +221,29 @@ def test_switch_off_matches_default(switch, method, n, o, ob, monkeypatch):
     (e0, t0), (e1, t1) = out
     np.testing.assert_allclose(e1, e0, rtol=1e-11, atol=0)
     assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
+
+
+def test_fp32_grouped_and_chained_close_to_fp64(monkeypatch):
+    """The FP32 forms of the grouped and chained launches (mixed precision):
+    one FP32 sweep at a size where both form stays within 1e-5 (relative to
+    the step) of the FP64 sweep and differs from it."""
+    _say("case fp32 grouped and chained")
+    c = Case(64, 8, 128, 47)
+    vec = _state(c, "CCSD", 17)
+    out = []
+    for prec in (0, 1):
+        eng = _engine_env(c, "CCSD", monkeypatch, {"CCQ_HIP_GRAPH": "0"}, shift=0.02)
+        eng.set_t(vec)
+        eng.set_precision(prec)
+        eng.profile(True)
+        eng.update()
+        rep = eng.profile_report()
+        eng.profile(False)
+        tags = [k.rsplit("/", 1)[1] for k in rep if k.startswith("gemm:")]
+        assert any(t.endswith("g") for t in tags) and any("c" in t for t in tags), sorted(rep)[:12]
+        out.append(eng.get_t())
+        eng.close()
+    step = np.abs(out[0] - vec).max()
+    diff = np.abs(out[1] - out[0]).max()
+    assert diff > 1e-13, "the FP32 contractions did not run"
+    assert diff < 1e-5 * step, (diff, step)
