@@ -1015,13 +1015,13 @@ struct ccq_hip_ctx {
     }
     fuse_adds(push32);
     // groups first; products left alone are chained where they can be
-    group_gemms();
+    group_gemms(false);
     chain_gemms(host32);
     for (auto& sec : sections) {
       sec.launches.clear();
       sec.launch_cls.clear();
     }
-    group_gemms();
+    group_gemms(dump_schedule);
     ws.alloc(std::max<int64_t>(ws_need, 1));
     if (lsq_n) lsq.alloc(lsq_n);
     HIPCHK(hipMalloc(&tables32, sizeof(int) * std::max<size_t>(host32.size(), 1)));
@@ -1255,7 +1255,7 @@ struct ccq_hip_ctx {
     return g;
   }
 
-  void group_gemms() {
+  void group_gemms(bool dump) {
     for (auto& sec : sections) {
       if (sec.kind != Section::Compute) continue;
       // launch units: ops, with a chain as its head
@@ -1345,7 +1345,7 @@ struct ccq_hip_ctx {
             grouped.insert(members.begin(), members.end());
           }
         }
-        if (dump_schedule) {
+        if (dump) {
           for (int i : here)
             std::fprintf(stderr, "[schedule] %s level %d op %d %s%s tiles %lld key %lld K %lld\n",
                          sec.name.c_str(), lv, i, sec.ops[i].cls.c_str(),
