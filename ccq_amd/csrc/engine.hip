@@ -1353,12 +1353,28 @@ struct ccq_hip_ctx {
                          (long long)ccqk::gemm_tiles(group_plan(sec.ops[i])),
                          (long long)ccqk::gemm_group_key(group_plan(sec.ops[i])), (long long)unit_k(sec, i));
         }
+        // the flat block copies of this level as one launch
+        {
+          std::vector<int> adds;
+          for (int i : here) {
+            const Op& op = sec.ops[i];
+            if (!op.gemm && op.flat && !grouped.count(i)) adds.push_back(i);
+          }
+          for (size_t lo = 0; adds.size() >= 2 && lo + 1 < adds.size(); lo += ccqk::kMaxAddGroup) {
+            const size_t hi = std::min(adds.size(), lo + (size_t)ccqk::kMaxAddGroup);
+            if (hi - lo < 2) break;
+            std::vector<int> members(adds.begin() + lo, adds.begin() + hi);
+            group_at[members[0]] = members;
+            grouped.insert(members.begin(), members.end());
+          }
+        }
         // this level in op order, each group where its first member was
         for (int i : here) {
           auto it = group_at.find(i);
           if (it != group_at.end()) {
             sec.launches.push_back(it->second);
-            sec.launch_cls.push_back(group_cls(sec, it->second));
+            sec.launch_cls.push_back(sec.ops[i].gemm ? group_cls(sec, it->second)
+                                                     : "add:flat*" + std::to_string(it->second.size()));
           } else if (!grouped.count(i)) {
             sec.launches.push_back({i});
             sec.launch_cls.push_back(sec.ops[i].chain >= 0 ? chain_cls(sec, sec.ops[i].chain) : sec.ops[i].cls);
@@ -1452,6 +1468,24 @@ struct ccq_hip_ctx {
       gs[k].group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, std::lround(std::sqrt(per_xcd))));
     }
     HIPCHK(ccqk::launch_gemm_group(stream, gs, n));
+  }
+
+  void run_add_group(const Section& sec, const std::vector<int>& members) {
+    ccqk::AddGroup ag{};
+    if (members.size() > (size_t)ccqk::kMaxAddGroup) throw Fail("grouped add launch too large");
+    ag.n = (int)members.size();
+    int64_t total = 0;
+    for (int k = 0; k < ag.n; ++k) {
+      const Op& op = sec.ops[members[k]];
+      ag.A[k] = tensors[op.a].ptr;
+      ag.C[k] = tensors[op.out].ptr;
+      ag.alpha[k] = op.coef;
+      ag.accumulate[k] = op.assign ? 0 : 1;
+      ag.start[k] = total;
+      total += op.M * op.N;
+    }
+    ag.start[ag.n] = total;
+    HIPCHK(ccqk::launch_axpy_group(stream, ag));
   }
 
   void run_op(const Op& op) {
@@ -1639,7 +1673,10 @@ struct ccq_hip_ctx {
             }
             double fl = 0, by = 0;
             for (int i : L) fl += sec.ops[i].flops, by += sec.ops[i].bytes;
-            timed(sec.launch_cls[k], fl, by, [&] { run_group(sec, L); });
+            if (sec.ops[L[0]].gemm)
+              timed(sec.launch_cls[k], fl, by, [&] { run_group(sec, L); });
+            else
+              timed(sec.launch_cls[k], fl, by, [&] { run_add_group(sec, L); });
           }
         }
       } else if (sec.kind == Section::Update) {

@@ -160,6 +160,19 @@ struct GemmChain {
   int n;
 };
 hipError_t launch_gemm_chain(hipStream_t s, const GemmArgs* es, int n);
+
+// Independent dense block copies / accumulations C_i (+)= alpha_i A_i over
+// identical layouts, in one launch (the small adds of one dependency level).
+constexpr int kMaxAddGroup = 16;
+struct AddGroup {
+  const double* A[kMaxAddGroup];
+  double* C[kMaxAddGroup];
+  double alpha[kMaxAddGroup];
+  int accumulate[kMaxAddGroup];
+  int64_t start[kMaxAddGroup + 1];   // first element of each member
+  int n;
+};
+hipError_t launch_axpy_group(hipStream_t s, const AddGroup& ag);
 hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
                            const int* colA, double* C, const int* rowC, const int* colC,
                            double alpha, int accumulate);

@@ -827,6 +827,22 @@ __global__ __launch_bounds__(256) void axpy_flat(int64_t n, const double* __rest
   }
 }
 
+// Several flat C_i (+)= alpha_i A_i in one launch: element e of the
+// concatenation belongs to the member whose range holds it.
+__global__ __launch_bounds__(256) void axpy_group(AddGroup ag) {
+  const int64_t total = ag.start[ag.n];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int m = 0;
+#pragma unroll
+    for (int i = 1; i < kMaxAddGroup; ++i)
+      if (i < ag.n && e >= ag.start[i]) m = i;
+    const int64_t k = e - ag.start[m];
+    const double v = ag.alpha[m] * ag.A[m][k];
+    if (ag.accumulate[m]) ag.C[m][k] += v; else ag.C[m][k] = v;
+  }
+}
+
 __global__ void copy_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x)
@@ -1627,6 +1643,14 @@ hipError_t launch_axpy_flat(hipStream_t s, int64_t n, const double* A, double* C
   if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(C)) & 15) return hipErrorInvalidValue;
   hipLaunchKernelGGL(axpy_flat, dim3(grid_for((n + 1) / 2, 256, 8192)), dim3(256), 0, s, n, A, C,
                      alpha, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_group(hipStream_t s, const AddGroup& ag) {
+  if (ag.n < 1 || ag.n > kMaxAddGroup || ag.start[0] != 0) return hipErrorInvalidValue;
+  const int64_t total = ag.start[ag.n];
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(axpy_group, dim3(grid_for(total, 256, 8192)), dim3(256), 0, s, ag);
   return hipGetLastError();
 }
 
