@@ -1,11 +1,13 @@
 """Build the gfx950 engine library in-tree: ccq_amd/libccq_hip.so.
 
 Steps: regenerate the embedded contraction IR from the spin-orbital programs
-(ccq_amd/gen), then compile the HIP sources with hipcc for gfx950.
+(ccq_amd/gen), compile each HIP source to an object with hipcc for gfx950
+(in parallel), then link the shared library.
 """
 
 from __future__ import annotations
 
+import concurrent.futures as cf
 import os
 import subprocess
 import sys
@@ -13,11 +15,12 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libccq_hip.so")
+OBJDIR = os.path.join(HERE, "_build")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("engine.hip", "kernels.hip", "guess.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # no packed FP32 (v_pk_*_f32) code generation anywhere in this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result",
@@ -30,15 +33,26 @@ def generate_ir(methods=None) -> None:
     emit.generate_all(os.path.join(HERE, "csrc"), methods or emit.METHODS)
 
 
-def build(verbose: bool = False, methods=None) -> str:
-    generate_ir(methods)
-    cmd = [HIPCC] + FLAGS + SOURCES + ["-o", LIB + ".tmp"]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libccq_hip.so")
+        raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+
+
+def build(verbose: bool = False, methods=None) -> str:
+    generate_ir(methods)
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = [os.path.join(OBJDIR, os.path.basename(s).replace(".hip", ".o")) for s in SOURCES]
+    with cf.ThreadPoolExecutor(len(SOURCES)) as ex:
+        futs = [ex.submit(_run, [HIPCC] + FLAGS + ["-c", s, "-o", o], verbose)
+                for s, o in zip(SOURCES, objs)]
+        for f in futs:
+            f.result()
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", LIB + ".tmp"],
+         verbose)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

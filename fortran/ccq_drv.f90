@@ -246,14 +246,16 @@ contains
         logical, intent(in) :: left
         real(dp), intent(out) :: e_out
         logical, intent(out) :: converged
-        real(dp) :: e_new, e_old, diff, res, conv(3), t0, t1
+        real(dp) :: e_new, e_old, diff, res, conv(3), t0, t1, nsec
         integer(c_int64_t) :: it
+        integer :: nmin
+        logical :: stop_file
         write(*, '(/2x,a4,3(a15),a16)') 'It.', 'E (Corr)', 'dE', 'Residuum', 'Wall Time'
         write(*, '(2x,65("-"))')
         conv = 1.0_dp
         e_old = 0.0_dp
         converged = .false.
-        call cpu_time(t0)
+        t0 = wall_time()
         do it = 1, max_iter
             if (left) then
                 call hip_check(ctx, ccq_hip_left_update(ctx), 'update_l2')
@@ -271,9 +273,23 @@ contains
                 write(*, '(a)') '      DIIS cycle'
                 call hip_check(ctx, ccq_hip_diis_extrapolate(ctx), 'calc_diis')
             end if
-            call cpu_time(t1)
-            write(*, '(2x,i4,3(f15.10),i5,'' min'',f5.1,'' s'')') it, e_new, diff, res, 0, t1 - t0
+            ! wall time of the iteration split into min:sec (print_iteration,
+            ! src/printing.F90:336-345; get_wall_time, src/utils/utils.F90:676)
+            t1 = wall_time()
+            nsec = t1 - t0
+            nmin = int(nsec) / 60
+            nsec = nsec - real(nmin, dp) * 60.0_dp
+            write(*, '(2x,i4,3(f15.10),i5,'' min'',f5.1,'' s'')') it, e_new, diff, res, nmin, nsec
             t0 = t1
+            ! external stop request: a file cc.comm is consumed and ends the
+            ! loop (check_sig, src/solve_cc.f90:246-249,271-291)
+            inquire(file='cc.comm', exist=stop_file)
+            if (stop_file) then
+                open(77, file='cc.comm', status='old')
+                close(77, status='delete')
+                write(*, '(a)') 'Signaled to end'
+                exit
+            end if
             if (all(abs(conv) < tol)) then
                 converged = .true.
                 exit
@@ -287,5 +303,12 @@ contains
         end do
         e_out = e_old
     end subroutine jacobi_iter
+
+    function wall_time() result(t)
+        real(dp) :: t
+        integer(c_int64_t) :: t_int, count_rate, count_max
+        call system_clock(t_int, count_rate, count_max)
+        t = real(t_int, dp) / real(count_rate, dp)
+    end function wall_time
 
 end program ccq_drv

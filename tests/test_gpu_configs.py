@@ -1,0 +1,265 @@
+"""The measured workloads at their full sizes on the GPU, through the C ABI.
+
+ccq's own regression tests are end-to-end energy checks of whole runs
+(tests/test_small.py:5-67).  These tests run each synthetic workload the
+engine is measured on at its real shape, where the brute-force oracle cannot
+go, and check what the mathematics fixes there:
+
+* CCSD, 100 orbitals / 10 occupied (B entries N(0, 0.01137^2), L = 200,
+  seed 1): one sweep and its energy from random amplitudes equal the
+  spin-integrated CCSD oracle's sweep (oracle/ccsd.py); at convergence the
+  oracle's residuals vanish;
+* CCSDT, 40 orbitals / 5 occupied (L = 80, seed 2): the first sweep from
+  T = 0 gives T2(aa) = MP2 exactly (T2A is updated first,
+  src/update_cc.F90:111-122); at the engine's converged amplitudes the
+  spin-orbital CCSDT oracle (oracle/ccsdt_so.py) finds every residual block
+  at zero and the same energy; all stored images are antisymmetric; the
+  closed-shell (rhf) sweep converges to the same energy; the mixed-precision
+  solve ends on the FP64 fixed point;
+* CCSDt + CC(t;3), 80 orbitals / 7 occupied, D2h irreps (act_occ 3,
+  act_unocc 8, L = 48, seed 3): symmetry-forbidden T1, T2, T3 and Q-space
+  T3 stay exactly zero, the packed t_vec has get_t_sizes_act's size, and the
+  H-bar / left-CC / crcc23 tail produces finite energies;
+* CCSDTQ, 20 orbitals / 4 occupied (L = 40, seed 4): MP2 first sweep, the
+  T1 residual of the CCSDT oracle (R1 has no T4 term) vanishes at the
+  converged amplitudes, T3 and T4 images are antisymmetric, rhf = uhf.
+"""
+
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from oracle import active as A
+from oracle import ccsdt_so
+from oracle import integrals as I
+from oracle import solver as S
+from ccq_amd import engine as E
+from ccq_amd import solve as SV
+from ccq_amd import synth
+import gpu_helpers as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _workload(name):
+    s = synth.workload_system(name)
+    e2 = s.e2int()
+    o = s.nocc
+    fa, fb = I.gen_fock(s.h, e2, o, o)
+    vaa, vab, vbb = I.antisymmetrize(e2, 0)
+    return s, (fa, fb, vaa, vab, vbb)
+
+
+def _engine(s, ints, method, **kw):
+    eng = E.Engine(0, s.nocc, s.nocc, s.norb, method, **kw)
+    eng.set_ints(*ints)
+    return eng
+
+
+def _assert_antisymmetric(blocks, names):
+    for k in names:
+        x = blocks[k]
+        sym = G.antisymmetrize_block(x, k)
+        assert np.abs(x - sym).max() <= 1e-12 * max(1.0, np.abs(x).max()), k
+
+
+def _mp2_t2aa(fa, vaa, o):
+    eo, ev = np.diag(fa)[:o], np.diag(fa)[o:]
+    D = ev[None, None, :, None] + ev[None, None, None, :] - eo[:, None, None, None] \
+        - eo[None, :, None, None]
+    # ccq's T2A step from T = 0: t = -R/D with R = <ab||ij> (t2aa[i,j,a,b])
+    v = vaa[o:, o:, :o, :o].transpose(2, 3, 0, 1)
+    return -v / D
+
+
+# --------------------------------------------------------------- CCSD n=100
+def test_ccsd_n100_sweep_matches_oracle():
+    _say("case ccsd-n100: one sweep + energy vs the CCSD oracle")
+    s, ints = _workload("ccsd-n100")
+    fa, fb, vaa, vab, vbb = ints
+    o, n = s.nocc, s.norb
+    v = n - o
+    rng = np.random.default_rng(41)
+    t = S.zero_amplitudes('CCSD', o, o, v, v)
+    for k in ('t1a', 't1b', 't2aa', 't2ab', 't2bb'):
+        t[k] = G.antisym_random(rng, k, o, o, v, v, 0.005)
+    eng = _engine(s, ints, "CCSD")
+    eng.set_t(E.join_tvec(t, o, o, v, v))
+    eng.update()
+    e = eng.energy()
+    got = E.split_tvec(eng.get_t(), o, o, v, v)
+    eng.close()
+    t0 = time.perf_counter()
+    D = S.block_denominators(t, fa, fb, o, o, 0.0)
+    S.sweep(t, S.sweep_order('CCSD', False), S.CCSDResiduals(s.h, s.e2int(), 0, o, o), D)
+    want_e = S.energy(fa, fb, vaa, vab, vbb, o, o, t)
+    _say(f"  oracle sweep {time.perf_counter() - t0:.1f} s")
+    for k in ('t1a', 't1b', 't2aa', 't2ab', 't2bb'):
+        scale = np.abs(t[k]).max()
+        assert np.abs(got[k] - t[k]).max() <= 1e-11 * scale, k
+    assert abs(e - want_e) < 1e-11
+
+
+def test_ccsd_n100_fixed_point_is_oracle_fixed_point():
+    _say("case ccsd-n100: converged amplitudes zero the oracle residuals")
+    s, ints = _workload("ccsd-n100")
+    fa, fb, vaa, vab, vbb = ints
+    o, n = s.nocc, s.norb
+    v = n - o
+    eng = _engine(s, ints, "CCSD")
+    res = SV.jacobi_iter(eng, tol=1e-11, max_iter=80, diis_space=5, out=None)
+    assert res.converged
+    t = E.split_tvec(eng.get_t(), o, o, v, v)
+    eng.close()
+    prov = S.CCSDResiduals(s.h, s.e2int(), 0, o, o)
+    D = S.block_denominators(t, fa, fb, o, o, 0.0)
+    for k in ('t1a', 't1b', 't2aa', 't2ab', 't2bb'):
+        step = np.abs(prov(k, t) / D[k]).max()
+        assert step < 1e-8, (k, step)
+    assert abs(S.energy(fa, fb, vaa, vab, vbb, o, o, t) - res.en_cor) < 1e-12
+
+
+# --------------------------------------------------------------- CCSDT n=40
+def test_ccsdt_n40_first_sweep_t2_is_mp2():
+    _say("case ccsdt-n40: first sweep from T = 0, T2(aa) = MP2")
+    s, ints = _workload("ccsdt-n40")
+    fa, fb, vaa, vab, vbb = ints
+    o, v = s.nocc, s.norb - s.nocc
+    eng = _engine(s, ints, "CCSDT")
+    eng.update()
+    t = E.split_tvec(eng.get_t(), o, o, v, v)
+    eng.close()
+    want = _mp2_t2aa(fa, vaa, o)
+    assert np.abs(t['t2aa'] - want).max() <= 1e-14 * np.abs(want).max() + 1e-16
+    _assert_antisymmetric(t, ('t2aa', 't2ab', 't2bb', 't3aaa', 't3aab', 't3abb', 't3bbb'))
+    assert np.abs(t['t3aab']).max() > 0
+
+
+def test_ccsdt_n40_fixed_point_is_oracle_fixed_point():
+    _say("case ccsdt-n40: converged amplitudes zero the spin-orbital CCSDT oracle")
+    s, ints = _workload("ccsdt-n40")
+    fa, fb, vaa, vab, vbb = ints
+    o, v = s.nocc, s.norb - s.nocc
+    eng = _engine(s, ints, "CCSDT")
+    res = SV.jacobi_iter(eng, tol=1e-11, max_iter=100, diis_space=5, out=None)
+    assert res.converged
+    _say(f"  engine: {res.iterations} iterations, E = {res.en_cor:.12f}")
+    t = E.split_tvec(eng.get_t(), o, o, v, v)
+    eng.close()
+    _assert_antisymmetric(t, ('t2aa', 't2ab', 't2bb', 't3aaa', 't3aab', 't3abb', 't3bbb'))
+    t0 = time.perf_counter()
+    R = ccsdt_so.Residuals(s.h, s.eri, o, o).block_residuals(t)
+    _say(f"  oracle residual {time.perf_counter() - t0:.1f} s")
+    D = S.block_denominators(t, fa, fb, o, o, 0.0)
+    for k, r in R.items():
+        step = np.abs(r / D[k]).max()
+        _say(f"  {k}: max |R/D| = {step:.2e}")
+        assert step < 1e-8, (k, step)
+    assert abs(S.energy(fa, fb, vaa, vab, vbb, o, o, t) - res.en_cor) < 1e-12
+
+
+def test_ccsdt_n40_rhf_equals_uhf_and_mixed_equals_fp64():
+    _say("case ccsdt-n40: rhf sweep, mixed precision fixed point")
+    s, ints = _workload("ccsdt-n40")
+    out = {}
+    for label, kw, mixed in (("uhf", {}, None), ("rhf", {"rhf": True}, None),
+                             ("mixed", {}, SV.Mixed(switch_de=1e-6))):
+        eng = _engine(s, ints, "CCSDT", **kw)
+        res = SV.jacobi_iter(eng, tol=1e-11, max_iter=100, diis_space=5, out=None, mixed=mixed)
+        assert res.converged, label
+        out[label] = (res.en_cor, eng.get_t())
+        _say(f"  {label}: {res.iterations} iterations, E = {res.en_cor:.12f}")
+        eng.close()
+    assert abs(out["rhf"][0] - out["uhf"][0]) < 1e-9
+    assert np.abs(out["rhf"][1] - out["uhf"][1]).max() < 1e-8
+    assert abs(out["mixed"][0] - out["uhf"][0]) < 1e-9
+    assert np.abs(out["mixed"][1] - out["uhf"][1]).max() < 1e-8
+
+
+# ------------------------------------------------------ CCSDt + CC(t;3) n=80
+def test_ccsdt_act_n80_d2h_zeros_and_cct3_tail():
+    _say("case ccsdt-act-n80: D2h and Q-space zeros, packed size, CC(t;3) tail")
+    s, ints = _workload("ccsdt-act-n80")
+    o, n = s.nocc, s.norb
+    v = n - o
+    ao, au = 3, 8
+    act = A.ActiveSpace(0, o, o, n, ao, au)
+    eng = E.Engine(0, o, o, n, "CC(t;3)", act_occ=ao, act_unocc=au)
+    eng.set_ints(*ints)
+    assert eng.t_size == 2 * o * v + 2 * (o * o * v * v) + (o * o * v * v) + \
+        A.packed_t3_size(act, o, o, v, v)
+    t0 = time.perf_counter()
+    res = SV.jacobi_iter(eng, tol=1e-7, max_iter=12, diis_space=5, out=None)
+    _say(f"  {res.iterations} CCSDt sweeps in {time.perf_counter() - t0:.1f} s, "
+         f"E = {res.en_cor:.10f}")
+    t = S.zero_amplitudes('CCSDT', o, o, v, v)
+    A.set_tvec(t, eng.get_t(), act, o, o, v, v)
+    irr = s.irreps
+    occ_irr, vir_irr = irr[:o], irr[o:]
+    for k in ('t1a', 't1b', 't2aa', 't2ab', 't2bb', 't3aaa', 't3aab', 't3abb', 't3bbb'):
+        rank = S.BLOCKS[k][0]
+        shape = t[k].shape
+        g = np.zeros(shape, dtype=np.int64)
+        ix = np.indices(shape)
+        for ax in range(2 * rank):
+            g ^= (occ_irr if ax < rank else vir_irr)[ix[ax]]
+        forbidden = g != 0
+        assert np.abs(t[k][~forbidden]).max() > 0, k
+        assert np.all(t[k][forbidden] == 0.0), k
+        if rank == 3:
+            q = ~A.p_mask(k, shape, act)
+            assert np.all(t[k][q] == 0.0), k
+    # the tail of src/calc_driver.f90:53-72 on the current amplitudes
+    eng.hbar()
+    left = SV.solve_lcc(eng, res.en_cor, 1e-7, 40, 5, out=None)
+    e23 = np.array(eng.cct3())
+    _say(f"  left-CC: {left.iterations} iterations, converged {left.converged}")
+    _say(f"  CC(t;3) corrections A-D: {e23}")
+    assert np.all(np.isfinite(e23)) and np.all(e23 < 0)
+    eng.close()
+
+
+# -------------------------------------------------------------- CCSDTQ n=20
+def test_ccsdtq_n20_properties():
+    _say("case ccsdtq-n20: MP2 first sweep, T1 oracle residual, antisymmetry, rhf = uhf")
+    s, ints = _workload("ccsdtq-n20")
+    fa, fb, vaa, vab, vbb = ints
+    o, v = s.nocc, s.norb - s.nocc
+    eng = _engine(s, ints, "CCSDTQ")
+    eng.update()
+    t = E.split_tvec(eng.get_t(), o, o, v, v)
+    want = _mp2_t2aa(fa, vaa, o)
+    assert np.abs(t['t2aa'] - want).max() <= 1e-14 * np.abs(want).max() + 1e-16
+    eng.set_t(np.zeros(eng.t_size))
+    eng.set_t4(np.zeros(eng.t4_size))
+    res = SV.jacobi_iter(eng, tol=1e-11, max_iter=100, diis_space=5, out=None)
+    assert res.converged
+    _say(f"  uhf: {res.iterations} iterations, E = {res.en_cor:.12f}")
+    t = E.split_tvec(eng.get_t(), o, o, v, v)
+    t4 = E.split_t4(eng.get_t4(), o, o, v, v)
+    eng.close()
+    _assert_antisymmetric(t, ('t2aa', 't2ab', 't2bb', 't3aaa', 't3aab', 't3abb', 't3bbb'))
+    _assert_antisymmetric(t4, ('t4aaaa', 't4aaab', 't4aabb', 't4abbb', 't4bbbb'))
+    assert np.abs(t4['t4aabb']).max() > 0
+    # R1 of CCSDTQ has no T4 term: the CCSDT oracle's R1 vanishes here
+    oracle = ccsdt_so.Residuals(s.h, s.eri, o, o)
+    t1so, t2so, t3so = oracle.to_so(t)
+    r1 = oracle.residuals(t1so, t2so, t3so, ranks=(1,))[1]
+    eo, ev = np.diag(fa)[:o], np.diag(fa)[o:]
+    d1 = np.concatenate([ev, ev])[None, :] - np.concatenate([eo, eo])[:, None]
+    step = np.abs(r1 / d1).max()
+    _say(f"  T1: max |R1/D| = {step:.2e}")
+    assert step < 1e-8
+    assert abs(S.energy(fa, fb, vaa, vab, vbb, o, o, t) - res.en_cor) < 1e-12
+    eng = _engine(s, ints, "CCSDTQ", rhf=True)
+    res_r = SV.jacobi_iter(eng, tol=1e-11, max_iter=100, diis_space=5, out=None)
+    eng.close()
+    assert res_r.converged
+    assert abs(res_r.en_cor - res.en_cor) < 1e-9

@@ -45,6 +45,8 @@ def _engine(c, method, monkeypatch, group, graph="0"):
 
 
 GROUPED = re.compile(r"^gemm:.*\*.*g$")
+# one launch of several flat block copies of a level (axpy_group)
+ADD_GROUP = re.compile(r"^add:flat\*(\d+)$")
 
 
 @pytest.mark.parametrize("method,n,o,ob", [("CCSD", 64, 8, 8), ("CCSD", 72, 9, 8), ("CCSDT", 30, 6, 6)])
@@ -61,11 +63,14 @@ def test_grouped_launches_match_single(method, n, o, ob, monkeypatch):
         rep = eng.profile_report()
         eng.profile(False)
         grouped = [k for k in rep if GROUPED.match(k)]
+        add_groups = [k for k in rep if ADD_GROUP.match(k)]
         if group == "1":
-            _say("  grouped classes: " + ", ".join(grouped[:6]))
+            _say("  grouped classes: " + ", ".join(grouped[:6] + add_groups[:4]))
             assert grouped, sorted(rep)[:20]
+            assert add_groups, sorted(rep)[:40]
+            assert all(2 <= int(ADD_GROUP.match(k).group(1)) <= 16 for k in add_groups)
         else:
-            assert not grouped
+            assert not grouped and not add_groups
         es = [eng.energy()]
         for _ in range(2):
             eng.update()
