@@ -1,33 +1,36 @@
 """Benchmark: time-to-|dE|<1e-7 Eh of the CC iteration on one MI355X,
-mixed precision against plain FP64.
+mixed precision plus the learned guess against plain FP64.
 
-Workload (N=1): closed-shell CCSD, 100 spatial orbitals (10 occupied,
-90 virtual), synthetic integrals (pq|rs) = sum_L B B with L = 200 from seed 1,
-tol-driven ccq iteration (diis_space 5, shift 0).
+Workload (N=1): CCSDT, 40 spatial orbitals (5 occupied, 35 virtual),
+synthetic integrals (pq|rs) = sum_L B B with L = 80 from seed 2 (B entries
+N(0, 0.05^2)), ccq's iteration (diis_space 5, shift 0, uhf sweep).  A second
+workload -- CCSD, 100 orbitals / 10 occupied, L = 200, seed 1, B entries
+N(0, 0.01137^2) -- is measured the same way and reported in
+config.secondary (--secondary none skips it).
 
-Protocol
+Protocol (per workload)
   1. build the system and upload it (untimed);
   2. W warm-up iterations from T = 0 in each precision, then a tight untimed
-     FP64 solve to get the converged energy E_conv and amplitudes (tol 1e-11);
-  3. plain leg: reset T = 0 and time exactly K FP64 iterations (update,
-     energy, DIIS ring, residuum, DIIS every 6th), synchronising the stream
-     around the timed region and after every iteration;
-  4. learned leg: the learned T2 guess (ccq_amd/guess_weights.npz; the MP2
-     guess if absent), computed inside the timed region, then K FP64
-     iterations;
-  5. mixed+learned leg: the learned guess, then K iterations with FP32-MFMA
-     contractions until |dE| < 1e-6 and FP64 after that (the refinement);
-  6. value = the mixed+learned leg's wall time until the first iteration
-     with |E - E_conv| < 1e-7; every leg's time, iteration count, final
-     energy and max-abs T error against the FP64 solution, and the speedup
-     over the plain leg go into config.
-  Roofline: the dominant GEMM class (the contraction shape with the largest
-  share of the sweep) timed with HIP events on the engine's stream during one
-  extra profiled iteration; algorithmic FLOPs 2*M*N*K per launch.
-  traffic: HBM bytes per launch of that class from the rocprofv3 FETCH_SIZE /
-  WRITE_SIZE runs summarised under profiles/ (copy-calibrated), when measured.
-  cpu_baseline: the numpy oracle (same equations, same sweep, OpenBLAS) on a
-  bounded sample of the same workload, extrapolated to the same iteration count.
+     FP64 solve (tol 1e-11) for the converged energy E_conv and amplitudes;
+  3. plain leg: T = 0, time exactly K FP64 iterations (update, energy, DIIS
+     ring, residuum, DIIS every 6th), stream synchronised after every
+     iteration;
+  4. learned leg: the learned starting guess (computed inside the timed
+     region; MP2 if no weights file), then K FP64 iterations;
+  5. mixed+learned leg: the learned guess, then FP32-MFMA contractions until
+     |dE| < 1e-6, FP64 after that; the iteration of the switch is recorded
+     and, as in solve.Mixed, only FP64 sweeps count as converged;
+  6. value = the mixed+learned leg's wall time until the first FP64
+     iteration with |E - E_conv| < 1e-7.
+  Roofline: the contraction class with the largest share of one profiled
+  FP64 sweep (HIP events on the engine's stream): 2*M*N*K flops per launch
+  over its average launch time.  traffic: HBM bytes per launch of that class
+  from the rocprofv3 FETCH_SIZE / WRITE_SIZE summaries under profiles/
+  (copy-calibrated), when one is recorded for it.
+  cpu_baseline: the CPU oracle (numpy / OpenBLAS on the host cores) on the
+  same system -- CCSDT: one evaluation of every residual block by the
+  spin-orbital oracle plus the Jacobi step; CCSD: the spin-integrated oracle
+  sweep -- timed and multiplied by the plain leg's iteration count.
 
 Prints one JSON line.
 """
@@ -52,45 +55,69 @@ from ccq_amd.inputs import build_integrals  # noqa: E402
 
 METRIC = "time-to-|dE|<1e-7 Eh, mixed+learned vs plain FP64 (1 GPU); % FP64 MFMA peak"
 PEAK_FP64_MFMA = 78.6e12
-PEAK_HBM = 8.0e12
 SWITCH_DE = 1e-6     # mixed leg: FP32 contractions until |dE| falls below this
+HIT = 1e-7
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_workload(name: str):
-    s = synth.workload_system(name)
-    ints = build_integrals(s.h, s.e2int(), s.enuc, 0, s.nocc, s.nocc)
-    return s, ints
+def workload_desc(name, rhf):
+    w = synth.WORKLOADS[name]
+    sigma = w.get("sigma", 0.05)
+    n, o = w["norb"], w["nocc"]
+    return (f"{name}: {w['method']}, n={n} o={o} v={n - o}, L={w['nL']}, seed {w['seed']}, "
+            f"B ~ N(0, {sigma:.5f}^2), diis 5, {'rhf' if rhf else 'uhf'} sweep")
 
 
-def run_iterations(eng, nsteps, diis_space, energies=None, stamps=None, switch_de=None):
-    """nsteps iterations of ccq's loop.  switch_de: mixed precision -- FP32
-    contractions until |dE| < switch_de, then FP64 (solve.Mixed's schedule)."""
-    en_cor = 0.0
-    low = switch_de is not None
-    if low:
-        eng.set_precision(1)
-    for it in range(1, nsteps + 1):
-        eng.update()
-        e = eng.energy()
-        if low and abs(en_cor - e) < switch_de:
-            low = False
-            eng.set_precision(0)
-        eng.diis_record(it)
-        eng.residuum(it)
-        if it % (diis_space + 1) == 0:
-            eng.diis_extrapolate()
-        if energies is not None:
-            energies.append(e)
-        if stamps is not None:
+class Leg:
+    """One timed run of ccq's loop from a fresh start."""
+
+    def __init__(self, eng, steps, diis, e_conv, t_conv):
+        self.eng, self.steps, self.diis = eng, steps, diis
+        self.e_conv, self.t_conv = e_conv, t_conv
+
+    def run(self, guess_kind=None, params=None, switch_de=None):
+        eng = self.eng
+        eng.set_t(np.zeros(eng.t_size))
+        if eng.t4_size:
+            eng.set_t4(np.zeros(eng.t4_size))
+        energies, stamps, fp64 = [], [], []
+        low = switch_de is not None
+        switch_it = None
+        eng.set_precision(1 if low else 0)
+        en_cor = 0.0
+        eng.sync()
+        start = time.perf_counter()
+        if guess_kind:
+            eng.guess(guess_kind, params)
+        for it in range(1, self.steps + 1):
+            fp64.append(not low)
+            eng.update()
+            e = eng.energy()
+            if low and abs(en_cor - e) < switch_de:
+                low = False
+                switch_it = it
+                eng.set_precision(0)
+            eng.diis_record(it)
+            eng.residuum(it)
+            if it % (self.diis + 1) == 0:
+                eng.diis_extrapolate()
             eng.sync()
             stamps.append(time.perf_counter())
-        en_cor = e
-    eng.set_precision(0)
-    return en_cor
+            energies.append(e)
+            en_cor = e
+        eng.set_precision(0)
+        total = time.perf_counter() - start
+        hit = next((i for i, e in enumerate(energies)
+                    if fp64[i] and abs(e - self.e_conv) < HIT), None)
+        t_end = eng.get_t()
+        return dict(total=total, value=(stamps[hit] - start) if hit is not None else None,
+                    n_needed=hit + 1 if hit is not None else None, switch=switch_it,
+                    e_err=abs(energies[-1] - self.e_conv),
+                    t_err=float(np.abs(t_end - self.t_conv).max()),
+                    fp32_sweeps=int(sum(1 for f in fp64 if not f)))
 
 
 def converge(eng, diis_space, tol=1e-11, max_iter=200):
@@ -110,33 +137,46 @@ def converge(eng, diis_space, tol=1e-11, max_iter=200):
     return en, max_iter
 
 
-def cpu_baseline(s, ints, n_iter_needed, budget_s=25.0):
-    """Oracle (numpy, OpenBLAS threads) sweeps on the same system; extrapolated."""
+def cpu_threads():
+    return int(os.environ.get("OPENBLAS_NUM_THREADS") or os.environ.get("OMP_NUM_THREADS")
+               or os.cpu_count() or 1)
+
+
+def cpu_baseline(s, ints, method, t_vec, n_iter):
+    """The CPU oracle on the same system, one sweep's worth of residual work,
+    times the plain leg's iteration count."""
     from oracle import solver as osolver
     o, n = s.nocc, s.norb
-    t = osolver.zero_amplitudes('CCSD', o, o, n - o, n - o)
-    prov = osolver.CCSDResiduals(s.h, ints.e2int, 0, o, o)
+    v = n - o
+    t = E.split_tvec(t_vec, o, o, v, v)
     D = osolver.block_denominators(t, ints.f_a, ints.f_b, o, o, 0.0)
-    seq = osolver.sweep_order('CCSD', False)
-    done = 0
-    t0 = time.perf_counter()
-    while True:
+    if method == "CCSDT":
+        from oracle import ccsdt_so
+        res = ccsdt_so.Residuals(s.h, s.eri, o, o)
+        t0 = time.perf_counter()
+        R = res.block_residuals(t)
+        for k in R:
+            t[k] = t[k] - R[k] / D[k]
+        osolver.energy(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb, o, o, t)
+        per = time.perf_counter() - t0
+        sample = (f"1 residual evaluation of every T1-T3 block by the spin-orbital CCSDT oracle "
+                  f"(oracle/ccsdt_so.py, numpy/OpenBLAS) + Jacobi step + energy on the same "
+                  f"system ({per:.1f} s), x {n_iter} iterations")
+    else:
+        prov = osolver.CCSDResiduals(s.h, ints.e2int, 0, o, o)
+        seq = osolver.sweep_order('CCSD', False)
+        t0 = time.perf_counter()
         osolver.sweep(t, seq, prov, D)
         osolver.energy(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb, o, o, t)
-        done += 1
-        if time.perf_counter() - t0 > budget_s or done >= 3:
-            break
-    per_iter = (time.perf_counter() - t0) / done
-    threads = int(os.environ.get("OPENBLAS_NUM_THREADS") or os.environ.get("OMP_NUM_THREADS")
-                  or os.cpu_count() or 1)
-    return dict(value=per_iter * n_iter_needed, unit="s", cores=threads, kind="port",
-                sample=f"{done} oracle CCSD sweep(s)+energy of the same system "
-                       f"({per_iter:.2f} s each, numpy/OpenBLAS), x {n_iter_needed} iterations")
+        per = time.perf_counter() - t0
+        sample = (f"1 spin-integrated CCSD oracle sweep + energy of the same system "
+                  f"({per:.1f} s, numpy/OpenBLAS), x {n_iter} iterations")
+    return dict(value=per * n_iter, unit="s", cores=cpu_threads(), kind="port", sample=sample)
 
 
 def measured_traffic(kernel_class: str):
-    """HBM bytes per launch of a GEMM class, from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs summarised under profiles/ (None if not measured)."""
+    """HBM bytes per launch of a kernel class from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summaries under profiles/ (None if not measured)."""
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None
@@ -162,13 +202,80 @@ def copy_probe(nbytes: int):
     log(f"copy probe: 3 x {nbytes} bytes")
 
 
+def measure(name, steps, warmup, rhf, with_cpu, profile=True):
+    diis = 5
+    method = synth.WORKLOADS[name]["method"]
+    t0 = time.perf_counter()
+    s = synth.workload_system(name)
+    ints = build_integrals(s.h, s.e2int(), s.enuc, 0, s.nocc, s.nocc)
+    o, n = s.nocc, s.norb
+    log(f"{name}: system built in {time.perf_counter() - t0:.1f}s (n={n} o={o} v={n - o})")
+    eng = E.Engine(0, o, o, n, method, rhf=rhf, diis_space=diis)
+    eng.set_ints(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb)
+    params, guess_kind = None, "mp2"
+    if os.path.exists(guess.WEIGHTS):
+        params, guess_kind = guess.load_weights(), "learned"
+    # warm-up (both precisions, the guess kernels) and the converged FP64 solution
+    warm = Leg(eng, warmup, diis, 0.0, 0.0)
+    warm.t_conv = np.zeros(eng.t_size)
+    warm.run()
+    warm.run(guess_kind, params, switch_de=0.0)
+    eng.set_t(np.zeros(eng.t_size))
+    if eng.t4_size:
+        eng.set_t4(np.zeros(eng.t4_size))
+    e_conv, it_conv = converge(eng, diis)
+    t_conv = eng.get_t()
+    log(f"{name}: E_conv = {e_conv:.12f} after {it_conv} iterations")
+    leg = Leg(eng, steps, diis, e_conv, t_conv)
+    legs = {}
+    for label, g, sw in (("plain", None, None), ("learned", guess_kind, None),
+                         ("mixed+learned", guess_kind, SWITCH_DE)):
+        r = leg.run(g, params, sw)
+        legs[label] = r
+        log(f"{name} {label}: {steps} iterations {r['total'] * 1e3 / steps:.3f} ms/iter; "
+            f"|E-E_conv|<1e-7 at iteration {r['n_needed']} ({r['value']} s); switch at "
+            f"{r['switch']}; final |dE| {r['e_err']:.2e}, max|dT| {r['t_err']:.2e}")
+    out = dict(workload=workload_desc(name, rhf), method=method, e_conv=e_conv, legs=legs,
+               guess=guess_kind)
+    if profile:
+        eng.set_t(t_conv)
+        eng.profile(True)
+        eng.update()
+        rep = eng.profile_report()
+        eng.profile(False)
+        gemms = {k: v for k, v in rep.items() if k.startswith("gemm:")}
+        dom_name, d = max(gemms.items(), key=lambda kv: kv[1]["ms"])
+        avg_s = d["ms"] / d["calls"] * 1e-3
+        fl_per = d["flops"] / d["calls"]
+        sweep_ms = sum(v["ms"] for v in rep.values())
+        gemm_ms = sum(v["ms"] for v in gemms.values())
+        gemm_fl = sum(v["flops"] for v in gemms.values())
+        log(f"{name}: dominant {dom_name}: {d['calls']} launches, avg {avg_s * 1e6:.1f} us, "
+            f"{fl_per / avg_s / 1e12:.2f} TF/s; all GEMMs {gemm_fl / 1e9:.1f} GF in "
+            f"{gemm_ms:.2f} ms ({gemm_fl / (gemm_ms * 1e-3) / 1e12:.2f} TF/s); profiled sweep "
+            f"{sweep_ms:.2f} ms")
+        for k, v in sorted(rep.items(), key=lambda kv: -kv[1]["ms"])[:24]:
+            log(f"  {k:40s} calls={v['calls']:4d} ms={v['ms']:.3f} "
+                f"avg_us={v['ms'] * 1e3 / v['calls']:.1f}")
+        out["roofline"] = dict(kernel=dom_name, avg_s=avg_s, flops=fl_per, sweep_ms=sweep_ms,
+                               gemm_tflops=gemm_fl / (gemm_ms * 1e-3) / 1e12, gemm_gf=gemm_fl / 1e9)
+    eng.close()
+    if with_cpu and legs["plain"]["n_needed"]:
+        log(f"{name}: cpu baseline (oracle) ...")
+        out["cpu"] = cpu_baseline(s, ints, method, t_conv, legs["plain"]["n_needed"])
+        log(f"{name}: cpu baseline {out['cpu']['value']:.1f} s ({out['cpu']['sample']})")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="ccsd-n100", choices=sorted(
+    ap.add_argument("--workload", default="ccsdt-n40", choices=sorted(
         k for k, v in synth.WORKLOADS.items() if v["method"] in ("CCSD", "CCSDT", "CCSDTQ")))
+    ap.add_argument("--secondary", default="ccsd-n100",
+                    help="second workload reported in config.secondary, or 'none'")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rhf", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -181,79 +288,36 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; sharding over several GPUs "
                                    "is designed but not built, so --gpus must be 1"}))
         sys.exit(2)
-    diis = 5
-    t0 = time.perf_counter()
-    s, ints = make_workload(args.workload)
-    o, n = s.nocc, s.norb
-    log(f"workload built in {time.perf_counter() - t0:.1f}s: n={n} o={o} v={n - o}")
-    method = synth.WORKLOADS[args.workload]["method"]
-    eng = E.Engine(0, o, o, n, method, rhf=args.rhf, diis_space=diis)
-    t0 = time.perf_counter()
-    eng.set_ints(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb)
-    log(f"upload {time.perf_counter() - t0:.1f}s")
-    zero = np.zeros(eng.t_size)
-    # warm-up (both precisions) and the converged FP64 reference (untimed)
-    eng.set_t(zero)
-    run_iterations(eng, args.warmup, diis)
-    eng.set_t(zero)
-    run_iterations(eng, args.warmup, diis, switch_de=0.0)
-    params, guess_kind = None, "mp2"
-    if os.path.exists(guess.WEIGHTS):
-        params, guess_kind = guess.load_weights(), "learned"
-    eng.guess(guess_kind, params)
-    eng.set_t(zero)
-    e_conv, it_conv = converge(eng, diis)
-    t_conv = eng.get_t()
-    log(f"E_conv = {e_conv:.12f} after {it_conv} iterations")
-    legs = {}
-    for leg, switch, start_guess in (("plain", None, None), ("learned", None, guess_kind),
-                                      ("mixed+learned", SWITCH_DE, guess_kind)):
-        eng.set_t(zero)
-        energies, stamps = [], []
-        eng.sync()
-        start = time.perf_counter()
-        if start_guess:
-            eng.guess(start_guess, params)
-        run_iterations(eng, args.steps, diis, energies, stamps, switch_de=switch)
-        eng.sync()
-        total = time.perf_counter() - start
-        hit = next((i for i, e in enumerate(energies) if abs(e - e_conv) < 1e-7), None)
-        t_end = eng.get_t()
-        legs[leg] = dict(total=total, value=(stamps[hit] - start) if hit is not None else None,
-                         n_needed=hit + 1 if hit is not None else None,
-                         e_err=abs(energies[-1] - e_conv),
-                         t2_err=float(np.abs(t_end - t_conv).max()))
-        log(f"{leg}: {args.steps} iterations {total * 1e3 / args.steps:.3f} ms/iter; "
-            f"|E-E_conv|<1e-7 at iteration {legs[leg]['n_needed']}: {legs[leg]['value']}; "
-            f"final |dE| {legs[leg]['e_err']:.2e}, max|dT| {legs[leg]['t2_err']:.2e}")
+    main_r = measure(args.workload, args.steps, args.warmup, args.rhf, not args.no_cpu)
+    sec = None
+    if args.secondary and args.secondary != "none" and args.secondary != args.workload:
+        sec = measure(args.secondary, args.steps, args.warmup, False, False, profile=True)
+    legs = main_r["legs"]
     value = legs["mixed+learned"]["value"]
-    n_needed = legs["plain"]["n_needed"]
-    total = legs["mixed+learned"]["total"]
-    # roofline of the dominant kernel: one profiled sweep (HIP events on the stream)
-    eng.profile(True)
-    eng.update()
-    rep = eng.profile_report()
-    eng.profile(False)
-    gemms = {k: v for k, v in rep.items() if k.startswith("gemm:")}
-    dom = max(gemms.items(), key=lambda kv: kv[1]["ms"])
-    dom_name, d = dom
-    avg_s = d["ms"] / d["calls"] * 1e-3
-    fl_per = d["flops"] / d["calls"]
-    achieved = fl_per / avg_s
-    sweep_ms = sum(v["ms"] for v in rep.values())
-    gemm_ms = sum(v["ms"] for v in gemms.values())
-    gemm_fl = sum(v["flops"] for v in gemms.values())
-    log(f"dominant {dom_name}: {d['calls']} launches, avg {avg_s * 1e6:.1f} us, "
-        f"{achieved / 1e12:.2f} TF/s; all GEMMs {gemm_fl / 1e9:.1f} GF in {gemm_ms:.2f} ms "
-        f"({gemm_fl / (gemm_ms * 1e-3) / 1e12:.2f} TF/s); profiled sweep {sweep_ms:.2f} ms")
-    top = sorted(rep.items(), key=lambda kv: -kv[1]["ms"])[:16]
-    for k, v in top:
-        log(f"  {k:36s} calls={v['calls']:4d} ms={v['ms']:.3f} avg_us={v['ms'] * 1e3 / v['calls']:.1f}")
-    cpu = None
-    if not args.no_cpu and n_needed is not None and method == "CCSD":
-        log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(s, ints, n_needed)
-        log(f"cpu baseline {cpu['value']:.1f} s ({cpu['sample']})")
+    rf = main_r["roofline"]
+    achieved = rf["flops"] / rf["avg_s"]
+
+    def summary(r):
+        lg = r["legs"]
+        out = {"workload": r["workload"], "e_conv": r["e_conv"],
+               "legs": {k: {"time_to_1e-7_s": v["value"], "iterations_to_1e-7": v["n_needed"],
+                            "ms_per_step": v["total"] * 1e3 / args.steps,
+                            "switch_to_fp64_at": v["switch"], "fp32_sweeps": v["fp32_sweeps"],
+                            "final_energy_error": v["e_err"], "final_max_abs_t_error": v["t_err"]}
+                        for k, v in lg.items()},
+               "speedup_vs_plain": (lg["plain"]["value"] / lg["mixed+learned"]["value"]
+                                    if lg["mixed+learned"]["value"] and lg["plain"]["value"]
+                                    else None),
+               "profiled_sweep_ms": r["roofline"]["sweep_ms"],
+               "all_gemm_tflops": r["roofline"]["gemm_tflops"]}
+        return out
+
+    cfg = summary(main_r)
+    cfg["mode"] = (f"mixed+learned: {main_r['guess']} starting guess, FP32-MFMA contractions until "
+                   f"|dE|<{SWITCH_DE:g}, then FP64; only FP64 sweeps count as converged")
+    if sec:
+        cfg["secondary"] = summary(sec)
+    cpu = main_r.get("cpu")
     out = {
         "metric": METRIC,
         "value": value,
@@ -261,35 +325,21 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": total * 1e3 / args.steps,
+        "ms_per_step": legs["mixed+learned"]["total"] * 1e3 / args.steps,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64/f32 mixed",
+        "dtype": "f64 (plain), f32/f64 (mixed)",
         "data": "synthetic (seeded integrals, ccq onebody/twobody construction)",
-        "config": {"workload": f"{args.workload}: {method}, n={n} o={o} v={n - o}, "
-                               f"L={synth.WORKLOADS[args.workload]['nL']}, seed "
-                               f"{synth.WORKLOADS[args.workload]['seed']}, diis 5, "
-                               f"{'rhf' if args.rhf else 'uhf'} sweep",
-                   "e_conv": e_conv,
-                   "mode": f"mixed+learned: {guess_kind} T2 guess, FP32-MFMA tiled contractions "
-                           f"until |dE|<{SWITCH_DE:g}, then FP64",
-                   "legs": {k: {"time_to_1e-7_s": v["value"], "iterations_to_1e-7": v["n_needed"],
-                                "ms_per_step": v["total"] * 1e3 / args.steps,
-                                "final_energy_error": v["e_err"],
-                                "final_max_abs_t_error": v["t2_err"]} for k, v in legs.items()},
-                   "speedup_vs_plain": (legs["plain"]["value"] / value
-                                        if value and legs["plain"]["value"] else None)},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA,
-                     "traffic": measured_traffic(dom_name),
-                     "kernel": dom_name, "avg_launch_us": avg_s * 1e6,
-                     "flops_per_launch": fl_per,
-                     "all_gemm_tflops": gemm_fl / (gemm_ms * 1e-3) / 1e12},
+        "config": cfg,
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12,
+                     "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_MFMA, "traffic": measured_traffic(rf["kernel"]),
+                     "kernel": rf["kernel"], "avg_launch_us": rf["avg_s"] * 1e6,
+                     "flops_per_launch": rf["flops"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    eng.close()
 
 
 if __name__ == "__main__":

@@ -17,7 +17,10 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libccq_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("engine.hip", "kernels.hip", "guess.hip")]
+# host-only sources (no device code): built with the host compiler
+HOST_SOURCES = [os.path.join(HERE, "csrc", f) for f in ("ints_io.cpp", "restart.cpp")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -46,13 +49,16 @@ def build(verbose: bool = False, methods=None) -> str:
     generate_ir(methods)
     os.makedirs(OBJDIR, exist_ok=True)
     objs = [os.path.join(OBJDIR, os.path.basename(s).replace(".hip", ".o")) for s in SOURCES]
-    with cf.ThreadPoolExecutor(len(SOURCES)) as ex:
+    hobjs = [os.path.join(OBJDIR, os.path.basename(s).replace(".cpp", ".o")) for s in HOST_SOURCES]
+    with cf.ThreadPoolExecutor(len(SOURCES) + len(HOST_SOURCES)) as ex:
         futs = [ex.submit(_run, [HIPCC] + FLAGS + ["-c", s, "-o", o], verbose)
                 for s, o in zip(SOURCES, objs)]
+        futs += [ex.submit(_run, [CXX, "-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-c", s,
+                                  "-o", o], verbose) for s, o in zip(HOST_SOURCES, hobjs)]
         for f in futs:
             f.result()
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", LIB + ".tmp"],
-         verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread"] + objs + hobjs +
+         ["-o", LIB + ".tmp"], verbose)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -22,6 +22,7 @@
 
 #include "../../include/ccq_hip.h"
 #include "kernels.h"
+#include "restart_io.h"
 
 #include "ir_ccsd_rhf.inc"
 #include "ir_ccsd_uhf.inc"
@@ -2340,6 +2341,75 @@ int ccq_hip_get_t4(ccq_hip_ctx* ctx, double* t4, int64_t n) {
     HIPCHK(hipMemcpyAsync(t4, ctx->tvec.p + ctx->t_size, sizeof(double) * n, hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+// ---- ccq's amplitude files (Fortran unformatted records, restart.cpp) ----
+
+int64_t ccq_hip_vec_size(const ccq_hip_ctx* ctx) { return ctx ? ctx->vec_size : -1; }
+
+int ccq_hip_write_vecs(ccq_hip_ctx* ctx, int64_t iter, int64_t diis_space, const char* dir) {
+  return guarded(ctx, [&] {
+    if (iter < 1 || diis_space < 1) throw Fail("write_vecs: iter and diis_space must be >= 1");
+    std::vector<double> t((size_t)ctx->ext_size);
+    if (ccq_hip_get_t(ctx, t.data(), ctx->ext_size) != 0) throw Fail(ctx->err);
+    int64_t slot = iter % (diis_space + 1);
+    if (slot == 0) slot = diis_space + 1;
+    std::string d = dir ? dir : "";
+    if (!d.empty() && d.back() != '/') d += '/';
+    for (const std::string& name : {std::string("t_vec.bin"),
+                                    "cc_iter_mat-" + std::to_string(slot) + ".bin"})
+      if (!ccqio::write_record(d + name, t.data(), ctx->vec_size))
+        throw Fail("write_vecs: cannot write " + d + name);
+  });
+}
+
+int ccq_hip_read_vec(ccq_hip_ctx* ctx, const char* path) {
+  return guarded(ctx, [&] {
+    std::vector<double> t((size_t)ctx->ext_size);
+    if (ccq_hip_get_t(ctx, t.data(), ctx->ext_size) != 0) throw Fail(ctx->err);
+    std::string why;
+    if (!ccqio::read_record(path ? path : "", t.data(), ctx->vec_size, why))
+      throw Fail("read_vec: " + why);
+    if (ccq_hip_set_t(ctx, t.data(), ctx->ext_size) != 0) throw Fail(ctx->err);
+  });
+}
+
+namespace {
+const char* const kT4Blocks[5] = {"t4aaaa", "t4aaab", "t4aabb", "t4abbb", "t4bbbb"};
+const char* const kT4Files[5] = {"_4a.bin", "_4b.bin", "_4c.bin", "_4d.bin", "_4e.bin"};
+}  // namespace
+
+int ccq_hip_write_t4(ccq_hip_ctx* ctx, const char* bin_file) {
+  return guarded(ctx, [&] {
+    const int64_t n = ccq_hip_t4_size(ctx);
+    if (n <= 0) throw Fail("write_t4: no T4 in this context (CCSDTQ only)");
+    std::vector<double> t4((size_t)n);
+    if (ccq_hip_get_t4(ctx, t4.data(), n) != 0) throw Fail(ctx->err);
+    int64_t off = 0;
+    for (int b = 0; b < 5; ++b) {
+      const int64_t sz = ctx->blk_size.at(kT4Blocks[b]);
+      const std::string path = std::string(bin_file ? bin_file : "") + kT4Files[b];
+      if (!ccqio::write_record(path, t4.data() + off, sz)) throw Fail("write_t4: cannot write " + path);
+      off += sz;
+    }
+  });
+}
+
+int ccq_hip_read_t4(ccq_hip_ctx* ctx, const char* bin_file) {
+  return guarded(ctx, [&] {
+    const int64_t n = ccq_hip_t4_size(ctx);
+    if (n <= 0) throw Fail("read_t4: no T4 in this context (CCSDTQ only)");
+    std::vector<double> t4((size_t)n);
+    int64_t off = 0;
+    std::string why;
+    for (int b = 0; b < 5; ++b) {
+      const int64_t sz = ctx->blk_size.at(kT4Blocks[b]);
+      const std::string path = std::string(bin_file ? bin_file : "") + kT4Files[b];
+      if (!ccqio::read_record(path, t4.data() + off, sz, why)) throw Fail("read_t4: " + why);
+      off += sz;
+    }
+    if (ccq_hip_set_t4(ctx, t4.data(), n) != 0) throw Fail(ctx->err);
   });
 }
 

@@ -49,6 +49,11 @@ EXPORTS = {
     "ccq_hip_t4_size": (ctypes.c_int64, [ctypes.c_void_p]),
     "ccq_hip_set_t4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_get_t4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_vec_size": (ctypes.c_int64, [ctypes.c_void_p]),
+    "ccq_hip_write_vecs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_char_p]),
+    "ccq_hip_read_vec": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "ccq_hip_write_t4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "ccq_hip_read_t4": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "ccq_hip_update": (ctypes.c_int, [ctypes.c_void_p]),
     "ccq_hip_energy": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "ccq_hip_diis_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
@@ -76,6 +81,17 @@ EXPORTS = {
                                         ctypes.c_int64, ctypes.c_void_p,
                                         ctypes.POINTER(ctypes.c_double)]),
     "ccq_hip_mlp_last_error": (ctypes.c_char_p, []),
+    "ccq_hip_onebody_orbitals": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+    "ccq_hip_read_onebody": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_void_p]),
+    "ccq_hip_read_twobody": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_fcidump_header": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
+                                              ctypes.POINTER(ctypes.c_int64)]),
+    "ccq_hip_read_fcidump": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_read_sym": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]),
+    "ccq_hip_read_last_error": (ctypes.c_char_p, []),
     "ccq_hip_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_profile_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "ccq_hip_debug_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
@@ -154,6 +170,21 @@ class Engine:
         out = np.empty(self.t4_size)
         self._check(self.lib.ccq_hip_get_t4(self.h, _ptr(out), out.size), "get_t4")
         return out
+
+    def write_vecs(self, it: int, diis_space: int, directory: str = "."):
+        """ccq's write_vecs: t_vec.bin and cc_iter_mat-<slot>.bin in ``directory``."""
+        self._check(self.lib.ccq_hip_write_vecs(self.h, it, diis_space, directory.encode()),
+                    "write_vecs")
+
+    def read_vec(self, path: str):
+        """Restart: the DIIS part of t_vec from a t_vec.bin file."""
+        self._check(self.lib.ccq_hip_read_vec(self.h, path.encode()), "read_vec")
+
+    def write_t4(self, bin_file: str):
+        self._check(self.lib.ccq_hip_write_t4(self.h, bin_file.encode()), "write_t4")
+
+    def read_t4(self, bin_file: str):
+        self._check(self.lib.ccq_hip_read_t4(self.h, bin_file.encode()), "read_t4")
 
     def update(self):
         self._check(self.lib.ccq_hip_update(self.h), "update")

@@ -118,7 +118,8 @@ def build(method: str, rhf: bool):
     res = {}
     for name in order:
         rank, nb = NAME_BLOCK[name]
-        res[name] = compile_section(prog, 'resid:' + name, stm_of[rank], [res_key(rank, nb)])
+        res[name] = compile_section(prog, 'resid:' + name, stm_of[rank], [res_key(rank, nb)],
+                                    merge_shared=6)
 
     def dressed_needs(names):
         need = set()

@@ -346,7 +346,7 @@ class Section:
 
 
 def compile_section(prog: Program, name: str, stmts, targets, fold_target=True,
-                    cond='always', live_kinds=('inter',)):
+                    cond='always', live_kinds=('inter',), merge_shared=0):
     """Spin-integrate ``stmts`` and keep what ``targets`` (BlockKeys) need.
 
     ``live_kinds`` are the tensor kinds this section computes itself; reads of
@@ -411,6 +411,8 @@ def compile_section(prog: Program, name: str, stmts, targets, fold_target=True,
         d = prog.decls[op.out.name]
         if fold_target and d.kind == 'resid':
             group = fold_images(d, group)
+            if merge_shared and d.rank >= merge_shared:
+                group = merge_shared_operand(prog, d, group, name)
         else:
             group = _merge(group)
         out.append((op.out, group))
@@ -426,6 +428,112 @@ def compile_section(prog: Program, name: str, stmts, targets, fold_target=True,
             first.add(op.out)
     return Section(name=name, kind='compute', ops=ordered, cond=cond,
                    extra={'targets': list(targets)})
+
+
+# sizes used to bound the merged operands (those of the factoriser)
+_MERGE_SIZES = {'o': 10, 'v': 70}
+_MERGE_MAX = 70 ** 3 * 10
+
+
+def _merge_signature(op, rkey, image, x):
+    """(shared operand key, its slot roles, the other operand's roles, labels)
+    of ``op`` relabelled by a residual image, with input ``x`` as the shared
+    operand.  Roles: ('o', p) output position p, ('s', k) k-th summed label in
+    the shared operand's order.  None when ``op`` is not a plain contraction."""
+    perm, sign = image
+    lc = op.out_labels
+    relabel = {lc[perm[p]]: lc[p] for p in range(len(lc))}
+    ins = [(k, "".join(relabel.get(c, c) for c in labs)) for k, labs in op.ins]
+    (kx, lx), (ky, ly) = ins[x], ins[1 - x]
+    if len(set(lx)) != len(lx) or len(set(ly)) != len(ly):
+        return None
+    summed = [c for c in lx if c not in lc]
+    if sorted(summed) != sorted(c for c in ly if c not in lc):
+        return None
+    if set(lx) | set(ly) != set(lc) | set(summed) or set(lx) & set(ly) & set(lc):
+        return None
+    role = {c: ('o', lc.index(c)) for c in lc}
+    role.update({c: ('s', k) for k, c in enumerate(summed)})
+    sig_x = tuple(role[c] for c in lx)
+    roles_y = tuple(role[c] for c in ly)
+    return (kx, sig_x), roles_y, sign, ins, x, role
+
+
+def merge_shared_operand(prog, decl, ops, section):
+    """Residual terms that read the same block through the same index mapping
+    (up to the residual's own antisymmetry images, R = A_G U) are merged:
+    their other operands are summed into one intermediate first, so the
+    block and the residual accumulator are streamed once for the group.
+
+      U += c1 X[..] Y1[..] + c2 X[..] Y2[..]   ->   N = c1 Y1 + c2 Y2;  U += X N
+
+    The shared operand is an amplitude block (T2, T3, T4); the merged
+    intermediate must be small (at most o v^3 at the factoriser's sizes)."""
+    cands = []
+    for op in ops:
+        best = None
+        if len(op.ins) == 2:
+            kinds = [prog.decls[k.name].kind for k, _ in op.ins]
+            xs = [i for i in (0, 1) if kinds[i] == 'amp' and kinds[1 - i] != 'amp'] or \
+                 [i for i in (0, 1) if kinds[i] == 'amp']
+            for x in xs:
+                for image in group_images(decl, op.out):
+                    sg = _merge_signature(op, op.out, image, x)
+                    if sg is None:
+                        continue
+                    ky = op.ins[1 - x][0]
+                    size = 1
+                    for c in ky.spaces:
+                        size *= _MERGE_SIZES[c]
+                    if size > _MERGE_MAX:
+                        continue
+                    key = (sg[0][0].tag(), sg[0][1], tuple(sorted(sg[1])))
+                    if best is None or key < best[0]:
+                        best = (key, sg)
+        cands.append(best)
+    groups = {}
+    for i, c in enumerate(cands):
+        if c is not None:
+            groups.setdefault(c[0], []).append(i)
+    merged = set()
+    out = []
+    count = [0]
+    for key, members in groups.items():
+        if len(members) < 2:
+            continue
+        (kx, sig_x), roles_y, _, _, x, _ = cands[members[0]][1]
+        ref = ops[members[0]]
+        # the merged operand's spaces / spins per role, in the first member's order
+        spaces, spins = [], []
+        for r in roles_y:
+            if r[0] == 'o':
+                spaces.append(ref.out.spaces[r[1]])
+                spins.append(ref.out.spins[r[1]])
+            else:
+                slot = sig_x.index(r)
+                spaces.append(kx.spaces[slot])
+                spins.append(kx.spins[slot])
+        count[0] += 1
+        mname = f"M{decl.rank // 2}{section.split(':')[-1]}_{count[0]}"
+        prog.declare(mname, len(roles_y), 0, (), kind='inter')
+        mkey = BlockKey(mname, "".join(spaces), tuple(spins))
+        first = True
+        for i in members:
+            op = ops[i]
+            _, ry, sign, ins, xi, role = cands[i][1]
+            inv = {v: c for c, v in role.items()}
+            nl = "".join(inv[r] for r in roles_y)
+            out.append(BlockOp(mkey, nl, op.coef * sign, (ins[1 - xi],), assign=first))
+            first = False
+            merged.add(i)
+        _, _, _, ins0, x0, role0 = cands[members[0]][1]
+        inv0 = {v: c for c, v in role0.items()}
+        nl0 = "".join(inv0[r] for r in roles_y)
+        pair = [None, None]
+        pair[x0] = ins0[x0]
+        pair[1 - x0] = (mkey, nl0)
+        out.append(BlockOp(ref.out, ref.out_labels, Fraction(1), tuple(pair)))
+    return [op for i, op in enumerate(ops) if i not in merged] + out
 
 
 def _merge(ops):

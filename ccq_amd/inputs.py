@@ -148,6 +148,17 @@ def parse_config(path: str) -> Config:
         # any other keyword is ignored, as in ccq
     if cfg.mult == 0:
         cfg.mult = 1
+    for attr in ("fcidump", "sym_file"):
+        p = getattr(cfg, attr)
+        if p and not os.path.isabs(p) and not os.path.exists(p):
+            cand = os.path.join(base, p)
+            if os.path.exists(cand):
+                setattr(cfg, attr, cand)
+    if cfg.fcidump:
+        # the FCIDUMP header overrides nel / nvir (src/parser.f90:168-169,258-311)
+        norb, nelec = fcidump_header(cfg.fcidump)
+        cfg.nel = nelec
+        cfg.nvirt = 2 * norb - nelec
     for attr in ("onebody", "twobody"):
         p = getattr(cfg, attr)
         if p and not os.path.isabs(p) and not os.path.exists(p):
@@ -157,32 +168,79 @@ def parse_config(path: str) -> Config:
     return cfg
 
 
+def _reader():
+    from .engine import load_library
+    return load_library()
+
+
+def _check(rc, lib, what):
+    if rc != 0:
+        raise ValueError(f"{what}: " + lib.ccq_hip_read_last_error().decode())
+
+
+def onebody_orbitals(path: str) -> int:
+    """Orbital count from the onebody file's n(n+1)/2 records (src/integrals.f90:248-251)."""
+    import ctypes
+    lib = _reader()
+    n = ctypes.c_int64()
+    _check(lib.ccq_hip_onebody_orbitals(path.encode(), ctypes.byref(n)), lib, "onebody")
+    return int(n.value)
+
+
 def read_onebody(path: str, norb: int) -> np.ndarray:
-    vals = np.loadtxt(path, usecols=0, converters={0: lambda s: float(s.replace(b"D", b"E") if isinstance(s, bytes) else s.replace("D", "E"))}, ndmin=1)
-    n = norb
-    if vals.size < n * (n + 1) // 2:
-        raise ValueError("onebody file too short for the orbital count")
-    h = np.zeros((n, n))
-    il = np.tril_indices(n)
-    # file order: i = 1..n, j = 1..i  (row-major lower triangle)
-    h[il] = vals[: n * (n + 1) // 2]
-    h = h + np.tril(h, -1).T
-    return h
+    """load_e1int (src/integrals.f90:11-41) through the library's reader."""
+    lib = _reader()
+    h = np.zeros((norb, norb), order="F")
+    _check(lib.ccq_hip_read_onebody(path.encode(), norb, h.ctypes.data), lib, "load_e1int")
+    return np.ascontiguousarray(h)
 
 
 def read_twobody(path: str, norb: int):
-    """Return (e2int[i,j,a,b] = value of line 'i a j b', E_nuc)."""
-    raw = np.loadtxt(path, ndmin=2)
-    idx = raw[:, :4].astype(np.int64)
-    end = np.nonzero(idx.sum(axis=1) == 0)[0]
-    enuc = 0.0
-    if end.size:
-        enuc = float(raw[end[0], 4])
-        raw, idx = raw[: end[0]], idx[: end[0]]
-    e2 = np.zeros((norb,) * 4)
-    i, a, j, b = (idx[:, k] - 1 for k in range(4))
-    e2[i, j, a, b] = raw[:, 4]
-    return e2, enuc
+    """load_e2int (src/integrals.f90:43-91): (e2int[i,j,a,b] = value of line
+    'i a j b', E_nuc), parsed in parallel by the library's reader."""
+    import ctypes
+    lib = _reader()
+    e2 = np.zeros((norb,) * 4, order="F")
+    enuc = ctypes.c_double(0.0)
+    _check(lib.ccq_hip_read_twobody(path.encode(), norb, e2.ctypes.data, ctypes.byref(enuc)),
+           lib, "load_e2int")
+    return np.ascontiguousarray(e2), float(enuc.value)
+
+
+def fcidump_header(path: str):
+    """(NORB, NELEC) of an FCIDUMP header (get_fcidump, src/parser.f90:258-311)."""
+    import ctypes
+    lib = _reader()
+    norb, nelec = ctypes.c_int64(), ctypes.c_int64()
+    _check(lib.ccq_hip_fcidump_header(path.encode(), ctypes.byref(norb), ctypes.byref(nelec)),
+           lib, "get_fcidump")
+    return int(norb.value), int(nelec.value)
+
+
+def read_fcidump(path: str, norb: int):
+    """load_fcidump (src/integrals.f90:93-210): (e1int, e2int[i,j,a,b], E_nuc)."""
+    import ctypes
+    lib = _reader()
+    h = np.zeros((norb, norb), order="F")
+    e2 = np.zeros((norb,) * 4, order="F")
+    enuc = ctypes.c_double(0.0)
+    _check(lib.ccq_hip_read_fcidump(path.encode(), norb, h.ctypes.data, e2.ctypes.data,
+                                     ctypes.byref(enuc)), lib, "load_fcidump")
+    return np.ascontiguousarray(h), np.ascontiguousarray(e2), float(enuc.value)
+
+
+def read_sym(path: str, norb: int):
+    """read_sym (src/symmetry.f90:15-119): (point group 1..4, irreps[orbs], 1-based)."""
+    import ctypes
+    lib = _reader()
+    pg = ctypes.c_int64()
+    irr = np.zeros(norb, dtype=np.int64)
+    _check(lib.ccq_hip_read_sym(path.encode(), norb, ctypes.byref(pg), irr.ctypes.data),
+           lib, "read_sym")
+    return int(pg.value), irr
+
+
+POINT_GROUPS = {1: "C1", 2: "CS", 3: "C2V", 4: "D2H"}
 
 
 @dataclasses.dataclass
@@ -219,7 +277,14 @@ def build_integrals(e1int, e2int, enuc, froz, occ_a, occ_b) -> Integrals:
 
 
 def load(cfg: Config) -> Integrals:
+    """load_ints (src/integrals.f90:212-298): FCIDUMP if given, else the
+    onebody / twobody pair (whose record count must give the orbital count)."""
     n = cfg.orbs
-    e1 = read_onebody(cfg.onebody, n)
-    e2, enuc = read_twobody(cfg.twobody, n)
+    if cfg.fcidump:
+        e1, e2, enuc = read_fcidump(cfg.fcidump, n)
+    else:
+        if onebody_orbitals(cfg.onebody) != n:
+            raise ValueError("load_ints: Number of orbitals doesn't match the number of integrals")
+        e1 = read_onebody(cfg.onebody, n)
+        e2, enuc = read_twobody(cfg.twobody, n)
     return build_integrals(e1, e2, enuc, cfg.froz, cfg.occ_a, cfg.occ_b)

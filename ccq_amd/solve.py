@@ -59,11 +59,28 @@ class Mixed:
     refine_every: int = 0
 
 
+def restart(eng: Engine, path: str = "t_vec.bin", bin_file: str | None = None,
+            out=sys.stdout) -> float:
+    """Resume from ccq's files: the DIIS vector from ``path`` (t_vec.bin) and,
+    for CCSDTQ, T4 from <bin_file>_4{a..e}.bin; prints ccq's restart line
+    (src/solve_cc.f90:56-60) and returns the restart correlation energy,
+    which seeds the first energy difference of the next jacobi_iter."""
+    eng.read_vec(path)
+    if bin_file is not None and eng.t4_size:
+        eng.read_t4(bin_file)
+    e = eng.energy()
+    if out:
+        print(f"  {'Restart correlation energy':>27s}  {e:16.10f}\n", file=out)
+    return e
+
+
 def jacobi_iter(eng: Engine, tol: float, max_iter: int, diis_space: int,
                 out=sys.stdout, on_iter=None, stop_file: str = "cc.comm",
-                mixed: Mixed | None = None) -> SolveResult:
+                mixed: Mixed | None = None, en_start: float = 0.0) -> SolveResult:
     conv = [1.0, 1.0, 1.0]
-    en_cor = 0.0
+    # conv%en_cor before iteration 1: ccq leaves it uninitialised
+    # (src/solver_types.f90:19) and sets it on restart (src/solve_cc.f90:57)
+    en_cor = en_start
     failed = False
     converged = False
     hist = []

@@ -102,6 +102,51 @@ int ccq_hip_residuum(ccq_hip_ctx* ctx, int64_t iter, double* res);
 /* DIIS extrapolation over the ring: calc_diis (src/diis.F90:11-84). */
 int ccq_hip_diis_extrapolate(ccq_hip_ctx* ctx);
 
+/* ---- integral input (host code; no GPU needed) ----
+ * Fortran column-major arrays with ccq's shapes; files are memory-mapped
+ * and parsed in parallel, with ccq's read semantics kept (see each call). */
+
+/* Orbital count from the number of records of a onebody file, n(n+1)/2:
+ * count_file_lines + load_ints (src/utils/utils.F90:622-649,
+ * src/integrals.f90:248-251). */
+int ccq_hip_onebody_orbitals(const char* path, int64_t* orbs);
+/* load_e1int (src/integrals.f90:11-41): e1int(orbs, orbs), symmetric. */
+int ccq_hip_read_onebody(const char* path, int64_t orbs, double* e1int);
+/* load_e2int (src/integrals.f90:43-91): e2int(orbs,orbs,orbs,orbs) =
+ * value of line "i a j b" at (i,j,a,b), never symmetrised; the line
+ * "0 0 0 0 E" gives en_repul and ends the read.  e2int must be zeroed by the
+ * caller (ccq zeroes it in load_ints, src/integrals.f90:240-244). */
+int ccq_hip_read_twobody(const char* path, int64_t orbs, double* e2int, double* en_repul);
+/* get_fcidump (src/parser.f90:258-311): NORB, NELEC of an FCIDUMP header. */
+int ccq_hip_fcidump_header(const char* path, int64_t* norb, int64_t* nelec);
+/* load_fcidump (src/integrals.f90:93-210): orbitals reordered by the
+ * energies on the "e i 0 0 0" lines (stable rank), 8-fold e2int fill,
+ * symmetric e1int, E_nuc.  e1int / e2int zeroed by the caller. */
+int ccq_hip_read_fcidump(const char* path, int64_t orbs, double* e1int, double* e2int,
+                         double* en_repul);
+/* read_sym (src/symmetry.f90:15-119): point group 1..4 (C1, Cs, C2v, D2h)
+ * and one irrep per orbital; a missing file gives C1, all irreps 1. */
+int ccq_hip_read_sym(const char* path, int64_t orbs, int64_t* point_group, int64_t* irreps);
+const char* ccq_hip_read_last_error(void);
+
+/* ---- ccq's amplitude files: Fortran unformatted sequential, one record of
+ * doubles (gfortran record markers) ---- */
+
+/* Length of the DIIS vector: conv%vec_size (src/solve_cc.f90:68-77). */
+int64_t ccq_hip_vec_size(const ccq_hip_ctx* ctx);
+/* write_vecs (src/diis.F90:289-349): the DIIS vector (the first vec_size
+ * values of t_vec) to <dir>/t_vec.bin and to the ring slot
+ * <dir>/cc_iter_mat-<k>.bin, k = mod(iter, diis_space+1) (0 -> diis_space+1). */
+int ccq_hip_write_vecs(ccq_hip_ctx* ctx, int64_t iter, int64_t diis_space, const char* dir);
+/* Restart: read a t_vec.bin written by ccq or by ccq_hip_write_vecs into the
+ * first vec_size values of t_vec (ccq's restart keyword, src/solve_cc.f90:
+ * 56-60, does not read it back; this does). */
+int ccq_hip_read_vec(ccq_hip_ctx* ctx, const char* path);
+/* T4 (CCSDTQ) to / from <bin_file>_4a.bin .. _4e.bin (open_t4_files,
+ * src/utils/cc_utils.f90:89-145), one block per file, ccq's layout. */
+int ccq_hip_write_t4(ccq_hip_ctx* ctx, const char* bin_file);
+int ccq_hip_read_t4(ccq_hip_ctx* ctx, const char* bin_file);
+
 /* ---- CC(t;3) tail (level 6), after the CCSDt iteration has converged ---- */
 
 /* One- and two-body H-bar from the resident T1, T2: replaces hbar2

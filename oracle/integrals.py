@@ -56,6 +56,104 @@ def load_e2int(path: str, norb: int) -> tuple[np.ndarray, float]:
     return e2, enuc
 
 
+def _fortran_real(tok: str) -> float:
+    return float(tok.replace("D", "E").replace("d", "e"))
+
+
+def fcidump_header(path: str):
+    """src/parser.f90:258-311 (get_fcidump) with parse_key_val
+    (src/utils/utils.F90:416-462): NORB and NELEC before the first line
+    holding END."""
+    norb = nelec = -1
+    with open(path) as f:
+        for line in f:
+            if "END" in line:
+                break
+            for key in ("NORB", "NELEC"):
+                k = line.find(key)
+                if k < 0:
+                    continue
+                rest = line[k:]
+                comma = rest.find(",")
+                eq = rest.find("=")
+                if eq < 0 or (comma >= 0 and eq > comma):
+                    continue
+                val = rest[eq + 1:comma] if comma >= 0 else rest[eq + 1:]
+                if key == "NORB":
+                    norb = int(val.split()[0])
+                else:
+                    nelec = int(val.split()[0])
+    return norb, nelec
+
+
+def load_fcidump(path: str, norb: int):
+    """src/integrals.f90:93-210.  Returns (e1int, e2int, E_nuc).
+
+    First pass: orbital energies from lines ``e i 0 0 0``; orbitals are
+    renumbered by a stable ascending rank of those energies
+    (insertion_rank, src/external/ranking.f90:84-140; energies absent from
+    the file count as 0).  Second pass: two-electron lines fill the eight
+    permutations of e2int(i,j,a,b) = (ia|jb), one-electron lines
+    e1int(i,a) = e1int(a,i), the all-zero line is E_nuc."""
+    body = []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    k = 0
+    while k < len(lines):
+        k += 1
+        if "END" in lines[k - 1]:
+            break
+    for line in lines[k:]:
+        parts = line.replace(",", " ").split()
+        if len(parts) < 5:
+            break
+        body.append((_fortran_real(parts[0]),) + tuple(int(x) for x in parts[1:5]))
+    eig = np.zeros(norb)
+    for v, i, a, j, b in body:
+        if i != 0 and a + j + b == 0:
+            eig[i - 1] = v
+    rank = list(range(norb))
+    for i in range(1, norb):          # stable insertion rank
+        tmp, j = rank[i], i - 1
+        while j >= 0 and not (eig[rank[j]] - eig[tmp] < 0.0):
+            rank[j + 1] = rank[j]
+            j -= 1
+        rank[j + 1] = tmp
+    inv = [0] * norb
+    for pos, orb in enumerate(rank):
+        inv[orb] = pos
+    e1 = np.zeros((norb, norb))
+    e2 = np.zeros((norb,) * 4)
+    enuc = 0.0
+    for v, i, a, j, b in body:
+        if i and a and j and b:
+            i, a, j, b = inv[i - 1], inv[a - 1], inv[j - 1], inv[b - 1]
+            for (p, q, r, s) in ((i, j, a, b), (j, i, b, a), (a, b, i, j), (b, a, j, i),
+                                 (a, j, i, b), (b, i, j, a), (i, b, a, j), (j, a, b, i)):
+                e2[p, q, r, s] = v
+        elif i and a and j + b == 0:
+            i, a = inv[i - 1], inv[a - 1]
+            e1[i, a] = v
+            e1[a, i] = v
+        elif i + a + j + b == 0:
+            enuc = v
+    return e1, e2, enuc
+
+
+def read_sym(path: str, norb: int):
+    """src/symmetry.f90:15-119: point group code (1 C1, 2 Cs, 3 C2v, 4 D2h)
+    then one irrep per orbital; a missing file is C1 with all irreps 1."""
+    import os
+    if not os.path.exists(path):
+        return 1, np.ones(norb, dtype=np.int64)
+    with open(path) as f:
+        toks = [ln.split()[0] for ln in f if ln.split()]
+    pg = int(toks[0])
+    if pg not in (1, 2, 3, 4):
+        raise ValueError("read_sym: ERROR: Point-group symmetry not supported")
+    return pg, np.array([int(x) for x in toks[1:norb + 1]], dtype=np.int64)
+
+
 def antisymmetrize(e2int: np.ndarray, froz: int):
     """src/integrals.f90:280-290 -- v_ab = e2int, v_aa = v_bb = e2int - e2int(i,j,b,a).
 
