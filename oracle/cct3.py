@@ -301,79 +301,107 @@ def tuple_denominators(H: Hbar, fso, sm: SpinMaps, occ, vir, spins_o, spins_v, d
     return np.array([da, db, dc, dc + dd_extra])
 
 
+def d3_extra(D, case, occ, vir):
+    """The d3 part of crcc23's D_D for one triple (src/mm_correct.f90:116-330):
+    sums of cal_d3's arrays over the triple's index pairs; ``case`` is the
+    number of beta indices (0 aaa, 1 aab, 2 abb, 3 bbb), occ = (i, j, k),
+    vir = (a, b, c) in the block's order."""
+    i, j, k = occ
+    a, b, c = vir
+    if case == 0:
+        return (D['d3a1'][a, j, i] + D['d3a1'][a, k, i] + D['d3a1'][a, k, j]
+                + D['d3a1'][b, j, i] + D['d3a1'][b, k, i] + D['d3a1'][b, k, j]
+                + D['d3a1'][c, j, i] + D['d3a1'][c, k, i] + D['d3a1'][c, k, j]
+                - D['d3a2'][b, a, i] - D['d3a2'][b, a, j] - D['d3a2'][b, a, k]
+                - D['d3a2'][c, a, i] - D['d3a2'][c, a, j] - D['d3a2'][c, a, k]
+                - D['d3a2'][c, b, i] - D['d3a2'][c, b, j] - D['d3a2'][c, b, k])
+    if case == 1:
+        return (D['d3a1'][a, j, i] + D['d3b1'][a, k, i] + D['d3b1'][a, k, j]
+                + D['d3a1'][b, j, i] + D['d3b1'][b, k, i] + D['d3b1'][b, k, j]
+                + D['d3c1'][c, k, i] + D['d3c1'][c, k, j]
+                - D['d3a2'][b, a, i] - D['d3a2'][b, a, j]
+                - D['d3b2'][c, a, i] - D['d3b2'][c, a, j] - D['d3c2'][c, a, k]
+                - D['d3b2'][c, b, i] - D['d3b2'][c, b, j] - D['d3c2'][c, b, k])
+    if case == 2:
+        return (D['d3b1'][a, j, i] + D['d3b1'][a, k, i]
+                + D['d3c1'][b, j, i] + D['d3c1'][b, k, i] + D['d3d1'][b, k, j]
+                + D['d3c1'][c, j, i] + D['d3c1'][c, k, i] + D['d3d1'][c, k, j]
+                - D['d3b2'][b, a, i] - D['d3c2'][b, a, j] - D['d3c2'][b, a, k]
+                - D['d3b2'][c, a, i] - D['d3c2'][c, a, j] - D['d3c2'][c, a, k]
+                - D['d3d2'][c, b, j] - D['d3d2'][c, b, k])
+    return (D['d3d1'][a, j, i] + D['d3d1'][a, k, i] + D['d3d1'][a, k, j]
+            + D['d3d1'][b, j, i] + D['d3d1'][b, k, i] + D['d3d1'][b, k, j]
+            + D['d3d1'][c, j, i] + D['d3d1'][c, k, i] + D['d3d1'][c, k, j]
+            - D['d3d2'][b, a, i] - D['d3d2'][b, a, j] - D['d3d2'][b, a, k]
+            - D['d3d2'][c, a, i] - D['d3d2'][c, a, j] - D['d3d2'][c, a, k]
+            - D['d3d2'][c, b, i] - D['d3d2'][c, b, j] - D['d3d2'][c, b, k])
+
+
+CASE_SPINS = {0: ((0, 0, 0), (0, 0, 0)), 1: ((0, 0, 1), (0, 0, 1)),
+              2: ((0, 1, 1), (0, 1, 1)), 3: ((1, 1, 1), (1, 1, 1))}
+
+
+def canonical_triples(sm: SpinMaps, case):
+    """(occ, vir) tuples of a spin case in crcc23's loop order: increasing
+    inside each same-spin run (i<j<k, a<b<c for aaa; i<j, a<b alpha and k, c
+    beta for aab; ...)."""
+    oa, ob, va, vb = sm.oa, sm.ob, sm.va, sm.vb
+    if case == 0:
+        for o in itertools.combinations(range(oa), 3):
+            for v in itertools.combinations(range(va), 3):
+                yield o, v
+    elif case == 1:
+        for i, j in itertools.combinations(range(oa), 2):
+            for k in range(ob):
+                for a, b in itertools.combinations(range(va), 2):
+                    for c in range(vb):
+                        yield (i, j, k), (a, b, c)
+    elif case == 2:
+        for i in range(oa):
+            for j, k in itertools.combinations(range(ob), 2):
+                for a in range(va):
+                    for b, c in itertools.combinations(range(vb), 2):
+                        yield (i, j, k), (a, b, c)
+    else:
+        for o in itertools.combinations(range(ob), 3):
+            for v in itertools.combinations(range(vb), 3):
+                yield o, v
+
+
+def in_p_space(sm: SpinMaps, case, occ, vir, m1, m2):
+    """crcc23's skip rules (src/mm_correct.f90:137,191,247,303): a triple of
+    the P space (an active occupied and an active virtual) is skipped."""
+    oa, ob = sm.oa, sm.ob
+    so, sv = CASE_SPINS[case]
+    orb_o = [x + 1 for x in occ]
+    orb_v = [(oa if s == 0 else ob) + x + 1 for s, x in zip(sv, vir)]
+    i, j, k = orb_o
+    a, b, c = orb_v
+    if case == 0 or case == 3:
+        return k > m1 and a <= m2
+    if case == 1:
+        return (j > m1 or k > m1) and (a <= m2 or c <= m2)
+    return (i > m1 or k > m1) and (a <= m2 or b <= m2)
+
+
 def correction(H: Hbar, l3so, m3so, fso, d3, sm: SpinMaps, m1: int, m2: int, by_case=False):
     """crcc23's four sums (src/mm_correct.f90:116-330), spin case by spin case,
     over the triples outside the P space; returns (e23a, e23b, e23c, e23d), or
     with by_case the 4 x 4 array [spin case aaa/aab/abb/bbb][A..D].
     Orbital numbers are 1-based with no frozen core; m1, m2 are
     sys%act_occ_b and sys%act_unocc_a."""
-    oa, ob, va, vb = sm.oa, sm.ob, sm.va, sm.vb
-    orb_o = lambda s, i: i + 1
-    orb_v = lambda s, a: (oa if s == 0 else ob) + a + 1
-    D = d3
     cases = np.zeros((4, 4))
-
-    def acc(occ, vir, spins_o, spins_v, dd_extra):
-        I = [sm.occ(s, x) for s, x in zip(spins_o, occ)]
-        A = [sm.vir(s, x) for s, x in zip(spins_v, vir)]
-        lm = l3so[I[0], I[1], I[2], A[0], A[1], A[2]] * m3so[I[0], I[1], I[2], A[0], A[1], A[2]]
-        den = tuple_denominators(H, fso, sm, occ, vir, spins_o, spins_v, dd_extra)
-        cases[sum(spins_o)] += lm / den
-
-    # A: alpha alpha alpha (i<j<k, a<b<c)
-    for i, j, k in itertools.combinations(range(oa), 3):
-        for a, b, c in itertools.combinations(range(va), 3):
-            if orb_o(0, k) > m1 and orb_v(0, a) <= m2:
+    for case in range(4):
+        spins_o, spins_v = CASE_SPINS[case]
+        for occ, vir in canonical_triples(sm, case):
+            if in_p_space(sm, case, occ, vir, m1, m2):
                 continue
-            x = (D['d3a1'][a, j, i] + D['d3a1'][a, k, i] + D['d3a1'][a, k, j]
-                 + D['d3a1'][b, j, i] + D['d3a1'][b, k, i] + D['d3a1'][b, k, j]
-                 + D['d3a1'][c, j, i] + D['d3a1'][c, k, i] + D['d3a1'][c, k, j]
-                 - D['d3a2'][b, a, i] - D['d3a2'][b, a, j] - D['d3a2'][b, a, k]
-                 - D['d3a2'][c, a, i] - D['d3a2'][c, a, j] - D['d3a2'][c, a, k]
-                 - D['d3a2'][c, b, i] - D['d3a2'][c, b, j] - D['d3a2'][c, b, k])
-            acc((i, j, k), (a, b, c), (0, 0, 0), (0, 0, 0), x)
-    # B: i<j alpha, k beta; a<b alpha, c beta
-    for i, j in itertools.combinations(range(oa), 2):
-        for k in range(ob):
-            for a, b in itertools.combinations(range(va), 2):
-                for c in range(vb):
-                    if (orb_o(0, j) > m1 or orb_o(1, k) > m1) and \
-                            (orb_v(0, a) <= m2 or orb_v(1, c) <= m2):
-                        continue
-                    x = (D['d3a1'][a, j, i] + D['d3b1'][a, k, i] + D['d3b1'][a, k, j]
-                         + D['d3a1'][b, j, i] + D['d3b1'][b, k, i] + D['d3b1'][b, k, j]
-                         + D['d3c1'][c, k, i] + D['d3c1'][c, k, j]
-                         - D['d3a2'][b, a, i] - D['d3a2'][b, a, j]
-                         - D['d3b2'][c, a, i] - D['d3b2'][c, a, j] - D['d3c2'][c, a, k]
-                         - D['d3b2'][c, b, i] - D['d3b2'][c, b, j] - D['d3c2'][c, b, k])
-                    acc((i, j, k), (a, b, c), (0, 0, 1), (0, 0, 1), x)
-    # C: i alpha, j<k beta; a alpha, b<c beta
-    for i in range(oa):
-        for j, k in itertools.combinations(range(ob), 2):
-            for a in range(va):
-                for b, c in itertools.combinations(range(vb), 2):
-                    if (orb_o(0, i) > m1 or orb_o(1, k) > m1) and \
-                            (orb_v(0, a) <= m2 or orb_v(1, b) <= m2):
-                        continue
-                    x = (D['d3b1'][a, j, i] + D['d3b1'][a, k, i]
-                         + D['d3c1'][b, j, i] + D['d3c1'][b, k, i] + D['d3d1'][b, k, j]
-                         + D['d3c1'][c, j, i] + D['d3c1'][c, k, i] + D['d3d1'][c, k, j]
-                         - D['d3b2'][b, a, i] - D['d3c2'][b, a, j] - D['d3c2'][b, a, k]
-                         - D['d3b2'][c, a, i] - D['d3c2'][c, a, j] - D['d3c2'][c, a, k]
-                         - D['d3d2'][c, b, j] - D['d3d2'][c, b, k])
-                    acc((i, j, k), (a, b, c), (0, 1, 1), (0, 1, 1), x)
-    # D: beta beta beta
-    for i, j, k in itertools.combinations(range(ob), 3):
-        for a, b, c in itertools.combinations(range(vb), 3):
-            if orb_o(1, k) > m1 and orb_v(1, a) <= m2:
-                continue
-            x = (D['d3d1'][a, j, i] + D['d3d1'][a, k, i] + D['d3d1'][a, k, j]
-                 + D['d3d1'][b, j, i] + D['d3d1'][b, k, i] + D['d3d1'][b, k, j]
-                 + D['d3d1'][c, j, i] + D['d3d1'][c, k, i] + D['d3d1'][c, k, j]
-                 - D['d3d2'][b, a, i] - D['d3d2'][b, a, j] - D['d3d2'][b, a, k]
-                 - D['d3d2'][c, a, i] - D['d3d2'][c, a, j] - D['d3d2'][c, a, k]
-                 - D['d3d2'][c, b, i] - D['d3d2'][c, b, j] - D['d3d2'][c, b, k])
-            acc((i, j, k), (a, b, c), (1, 1, 1), (1, 1, 1), x)
+            I = [sm.occ(s, x) for s, x in zip(spins_o, occ)]
+            A = [sm.vir(s, x) for s, x in zip(spins_v, vir)]
+            lm = l3so[I[0], I[1], I[2], A[0], A[1], A[2]] * m3so[I[0], I[1], I[2], A[0], A[1], A[2]]
+            den = tuple_denominators(H, fso, sm, occ, vir, spins_o, spins_v,
+                                     d3_extra(d3, case, occ, vir))
+            cases[case] += lm / den
     if by_case:
         return cases
     return tuple(cases.sum(axis=0))

@@ -16,8 +16,12 @@
 * crcc23's sums: closed-shell spin symmetry case by case (aaa = bbb,
   aab = abb) and no correction when every triple is in the P space.
 
-The left-CC equations are pinned at CCSD amplitudes; with CCSDt amplitudes
-ccq solves the same equations (H-bar from T1, T2), which no brute force fixes.
+* D_D with three-body terms: with T1 = 0 it equals minus the brute-force
+  diagonal <ijk abc|Hbar|ijk abc> - <0|Hbar|0> for every triple (the loop
+  quirk switched off), D_C does not;
+* the left equations at arbitrary (CCSDt) T1, T2 and L: the exact
+  <0|(1+L)(Hbar-E)|mu> minus the oracle residual is zero on doubles and the
+  disconnected sum_jb l_ij^ab R_j^b on singles.
 """
 
 import itertools
@@ -221,3 +225,105 @@ def test_correction_closed_shell_spin_symmetry_and_full_p_space():
     assert np.abs(cases).max() > 1e-7
     full = A.ActiveSpace(0, oa, oa, n, oa, n - oa)
     assert C.correction(H, l3, R[3], fso, d3, sm, full.m1, full.m2) == (0.0, 0.0, 0.0, 0.0)
+
+
+def _random_so(space, rng, rank, scale):
+    from oracle.ccsdt_so import _anti
+    No, Nv = space.nocc_so, space.nvir_so
+    x = scale * rng.standard_normal((No,) * rank + (Nv,) * rank)
+    x = _anti(x, [list(range(rank)), list(range(rank, 2 * rank))])
+    return bf.blocks_to_so(space, rank, bf.so_to_blocks(space, rank, x))
+
+
+@pytest.mark.parametrize("n,oa,ob", [(6, 3, 3), (6, 3, 2)])
+def test_dd_is_brute_force_triples_diagonal(n, oa, ob):
+    """With T1 = 0 the normal-ordered H-bar has no rank-0 piece beyond three
+    bodies, so crcc23's D_D (one-, two- and the d3 three-body diagonal,
+    src/mm_correct.f90:138-152,346-481, loop-start quirk off) must equal
+    -(<mu|Hbar|mu> - <0|Hbar|0>) for every triple of every spin case; D_C
+    (no d3) must not.  A wrong index pairing inside cal_d3 fails this."""
+    s = synth.make_system(n, oa, 20, 5)
+    space = bf.DetSpace(n, oa, ob)
+    no, nv = space.nocc_so, space.nvir_so
+    hso, vso = C.so_integrals(space, s.h, s.eri)
+    Hm = bf.hamiltonian(space, s.h, s.eri)
+    rng = np.random.default_rng(4)
+    t1 = np.zeros((no, nv))
+    t2 = _random_so(space, rng, 2, 0.1)
+    H = C.Hbar(*C.dress(hso, vso, t1), t2, no)
+    fso = C.fock(hso, vso, no)
+    Hb = _similarity(space, Hm, {1: t1, 2: t2})
+    r0 = space.index[space.ref]
+    sm = C.SpinMaps(n, oa, ob)
+    d3 = C.d3_arrays(H, vso[:no, :no, no:, no:], t2, sm, d3d1_start=0)
+    # mistakes the identity must expose: the alpha-alpha array used for the
+    # alpha-beta one, and (ccq's loop-start quirk) the first beta virtual
+    # dropped from d3d1's sum
+    bad = dict(d3)
+    bad['d3b1'] = d3['d3a1'][:, :ob, :].copy()
+    quirk = C.d3_arrays(H, vso[:no, :no, no:, no:], t2, sm, d3d1_start=1)
+    gap_c = gap_bad = gap_quirk = 0.0
+    count = 0
+    for case in range(4):
+        so_, sv_ = C.CASE_SPINS[case]
+        for occ, vir in C.canonical_triples(sm, case):
+            I = tuple(sm.occ(a, x) for a, x in zip(so_, occ))
+            A = tuple(sm.vir(a, x) for a, x in zip(sv_, vir))
+            _, d = bf._apply(bf._exc_ops(space, I, A), space.ref)
+            diag = Hb[space.index[d], space.index[d]] - Hb[r0, r0]
+            den = C.tuple_denominators(H, fso, sm, occ, vir, so_, sv_,
+                                       C.d3_extra(d3, case, occ, vir))
+            assert abs(den[3] + diag) < 1e-12, (case, occ, vir)
+            gap_c = max(gap_c, abs(den[2] + diag))
+            gap_bad = max(gap_bad, abs(C.d3_extra(bad, case, occ, vir)
+                                       - C.d3_extra(d3, case, occ, vir)))
+            gap_quirk = max(gap_quirk, abs(C.d3_extra(quirk, case, occ, vir)
+                                           - C.d3_extra(d3, case, occ, vir)))
+            count += 1
+    assert count > 50
+    assert gap_c > 1e-3 and gap_bad > 1e-3 and gap_quirk > 1e-3
+
+
+@pytest.mark.parametrize("n,oa,ob", [(6, 3, 3), (6, 3, 2)])
+def test_left_residual_at_nonstationary_t(n, oa, ob):
+    """At any T1, T2 (not a CCSD solution: ccq solves the left equations at
+    CCSDt amplitudes) and any L1, L2, the exact <0|(1+L)(Hbar-E)|mu>
+    (brute force) equals the oracle's left residual -- ccq's connected terms
+    plus the l1 x h_ov product of compute_l_hbar_disconnected
+    (src/update_lcc.f90:106-206) -- on doubles, and on singles exceeds it by
+    exactly the disconnected products sum_jb l_ij^ab R_j^b, R_j^b =
+    <jb|Hbar|0>, which ccq does not include (zero at CCSD convergence)."""
+    s = synth.make_system(n, oa, 20, 5)
+    space = bf.DetSpace(n, oa, ob)
+    no, nv = space.nocc_so, space.nvir_so
+    hso, vso = C.so_integrals(space, s.h, s.eri)
+    Hm = bf.hamiltonian(space, s.h, s.eri)
+    rng = np.random.default_rng(8)
+    t1, t2 = _random_so(space, rng, 1, 0.05), _random_so(space, rng, 2, 0.1)
+    l1, l2 = _random_so(space, rng, 1, 0.1), _random_so(space, rng, 2, 0.1)
+    H = C.Hbar(*C.dress(hso, vso, t1), t2, no)
+    Hb = _similarity(space, Hm, {1: t1, 2: t2})
+    r0 = space.index[space.ref]
+    E = Hb[r0, r0]
+    basis = _sd_basis(space)
+    lam = np.array([l1[oc[0], vi[0]] if rk == 1 else l2[oc + vi] for rk, oc, vi, _, _ in basis])
+    R1 = np.zeros((no, nv))
+    for rk, oc, vi, sg, d in basis:
+        if rk == 1:
+            R1[oc[0], vi[0]] = sg * Hb[d, r0]
+    assert np.abs(R1).max() > 1e-2
+    extra = np.einsum('ijab,jb->ia', l2, R1)
+    r1, r2 = C.lambda_residual(H, t2, l1, l2)
+    r1_bad, _ = C.lambda_residual(H, 0 * t2, l1, l2)     # Gae / Gmi dropped
+    worst_bad = 0.0
+    for m, (rk, oc, vi, sm_, dm) in enumerate(basis):
+        brute = Hb[r0, dm] * sm_
+        for k, (_, _, _, sn, dn) in enumerate(basis):
+            brute += lam[k] * sn * sm_ * (Hb[dn, dm] - (E if dn == dm else 0.0))
+        if rk == 1:
+            i, a = oc[0], vi[0]
+            assert abs(brute - (r1[i, a] + extra[i, a])) < 1e-12
+            worst_bad = max(worst_bad, abs(brute - (r1_bad[i, a] + extra[i, a])))
+        else:
+            assert abs(brute - r2[oc + vi]) < 1e-12
+    assert worst_bad > 1e-4
