@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -359,6 +360,19 @@ struct DevF {
     }                                                                      \
   } while (0)
 
+// loss and gradient over all n samples (device buffers; scr holds n * SCR
+// floats, grad kGuessParams + 1), on the null stream
+hipError_t launch_full_grad(const float* P, const float* X, const float* M, const float* Y,
+                            int64_t n, float* scr, float* grad) {
+  using namespace ccqk;
+  hipLaunchKernelGGL(mlp_fwd_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, P, X, M, Y,
+                     n, n, (uint64_t)0, (int64_t)0, 1, scr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(mlp_grad, dim3((kGuessParams + 1 + 255) / 256), dim3(256), 0, 0, scr, n, grad);
+  return hipGetLastError();
+}
+
 int run_training(const float* X, const float* M, const float* Y, int64_t n, float* params,
                  int64_t nparams, int64_t steps, int64_t batch, double lr, uint64_t seed,
                  double* loss_out, float* grad_out) {
@@ -383,16 +397,19 @@ int run_training(const float* X, const float* M, const float* Y, int64_t n, floa
   GCHK(hipMemcpy(dP.p, params, sizeof(float) * kGuessParams, hipMemcpyHostToDevice));
   GCHK(hipMemset(dm1.p, 0, sizeof(float) * kGuessParams));
   GCHK(hipMemset(dm2.p, 0, sizeof(float) * kGuessParams));
-  const dim3 gb((unsigned)((batch + 255) / 256)), gp((kGuessParams + 1 + 255) / 256);
-  const int64_t nsteps = grad_out ? 1 : steps;
+  const dim3 gp((kGuessParams + 1 + 255) / 256);
+  if (grad_out) {
+    GCHK(launch_full_grad(dP.p, dX.p, dM.p, dY.p, n, dS.p, dG.p));
+    steps = 0;
+  }
+  const dim3 gb((unsigned)((batch + 255) / 256));
   double b1 = 1.0, b2 = 1.0;
-  for (int64_t st = 0; st < nsteps; ++st) {
+  for (int64_t st = 0; st < steps; ++st) {
     hipLaunchKernelGGL(mlp_fwd_bwd, gb, dim3(256), 0, 0, dP.p, dX.p, dM.p, dY.p, n, batch, seed, st,
-                       grad_out ? 1 : 0, dS.p);
+                       0, dS.p);
     GCHK(hipGetLastError());
     hipLaunchKernelGGL(mlp_grad, gp, dim3(256), 0, 0, dS.p, batch, dG.p);
     GCHK(hipGetLastError());
-    if (grad_out) break;
     b1 *= 0.9;
     b2 *= 0.999;
     hipLaunchKernelGGL(mlp_adam, gp, dim3(256), 0, 0, dP.p, dG.p, dm1.p, dm2.p, (float)lr,
@@ -408,9 +425,87 @@ int run_training(const float* X, const float* M, const float* Y, int64_t n, floa
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Gradient without device allocation: every device buffer of the gradient
+// (the staged samples and parameters, the per-sample activations and
+// signals, the gradient and loss) lives in one file-scope array.  Each GPU
+// holds its own copy until the process ends; its address is looked up once
+// per device and used for every kernel argument and copy.
+
+constexpr int64_t kNaMax = CCQ_HIP_MLP_NOALLOC_MAX_SAMPLES;
+constexpr int64_t kNaX = 0;                                   // X[n][F]
+constexpr int64_t kNaM = kNaX + kNaMax * ccqk::F;             // M[n]
+constexpr int64_t kNaY = kNaM + kNaMax;                       // Y[n]
+constexpr int64_t kNaP = kNaY + kNaMax;                       // params
+constexpr int64_t kNaG = kNaP + 64 * ((ccqk::kGuessParams + 63) / 64);        // grad + loss
+constexpr int64_t kNaS = kNaG + 64 * ((ccqk::kGuessParams + 1 + 63) / 64);    // scr[n][SCR]
+constexpr int64_t kNaFloats = kNaS + kNaMax * ccqk::SCR;
+
+constexpr int kNaDevices = 64;
+std::mutex g_na_mu;
+float* g_na_ptr[kNaDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) float scratch[kNaFloats];
+
+namespace {
+
+// the device address of this GPU's scratch (looked up on first use)
+float* noalloc_scratch(int dev) {
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_guess_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_na_ptr[dev] = static_cast<float*>(p);
+  }
+  return g_na_ptr[dev];
+}
+
 }  // namespace
 
 extern "C" {
+
+int ccq_hip_mlp_grad_noalloc(const float* X, const float* M, const float* Y, int64_t n,
+                             const float* params, int64_t nparams, float* grad, double* loss) {
+  using namespace ccqk;
+  if (nparams != kGuessParams) {
+    g_guess_err = "nparams must be " + std::to_string(kGuessParams);
+    return 2;
+  }
+  if (n <= 0 || n > kNaMax) {
+    g_guess_err = "sample count must be in 1.." + std::to_string(kNaMax) +
+                  " (CCQ_HIP_MLP_NOALLOC_MAX_SAMPLES) for ccq_hip_mlp_grad_noalloc";
+    return 2;
+  }
+  if (!X || !M || !Y || !params || !grad) {
+    g_guess_err = "null array";
+    return 2;
+  }
+  int dev = 0;
+  GCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaDevices) {
+    g_guess_err = "device ordinal beyond the scratch table";
+    return 2;
+  }
+  float* s = noalloc_scratch(dev);
+  if (!s) return 1;
+  GCHK(hipMemcpy(s + kNaX, X, sizeof(float) * n * F, hipMemcpyHostToDevice));
+  GCHK(hipMemcpy(s + kNaM, M, sizeof(float) * n, hipMemcpyHostToDevice));
+  GCHK(hipMemcpy(s + kNaY, Y, sizeof(float) * n, hipMemcpyHostToDevice));
+  GCHK(hipMemcpy(s + kNaP, params, sizeof(float) * kGuessParams, hipMemcpyHostToDevice));
+  GCHK(launch_full_grad(s + kNaP, s + kNaX, s + kNaM, s + kNaY, n, s + kNaS, s + kNaG));
+  float l = 0.0f;
+  GCHK(hipMemcpy(&l, s + kNaG + kGuessParams, sizeof(float), hipMemcpyDeviceToHost));
+  GCHK(hipMemcpy(grad, s + kNaG, sizeof(float) * kGuessParams, hipMemcpyDeviceToHost));
+  GCHK(hipDeviceSynchronize());
+  if (loss) *loss = l;
+  return 0;
+}
 
 int64_t ccq_hip_mlp_nparams(void) { return ccqk::kGuessParams; }
 

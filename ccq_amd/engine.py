@@ -80,6 +80,9 @@ EXPORTS = {
     "ccq_hip_mlp_grad": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_int64, ctypes.c_void_p,
                                         ctypes.c_int64, ctypes.c_void_p,
                                         ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_mlp_grad_noalloc": (ctypes.c_int, [ctypes.c_void_p] * 3 + [ctypes.c_int64, ctypes.c_void_p,
+                                        ctypes.c_int64, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_double)]),
     "ccq_hip_mlp_last_error": (ctypes.c_char_p, []),
     "ccq_hip_onebody_orbitals": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     "ccq_hip_read_onebody": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.c_void_p]),

@@ -108,7 +108,9 @@ def train(X, M, Y, params, steps=3000, batch=4096, lr=3e-3, seed=1):
     return p, loss.value
 
 
-def grad(X, M, Y, params):
+def grad(X, M, Y, params, noalloc=False):
+    """Loss and gradient over all samples; noalloc selects the entry point
+    that uses the library's static per-GPU scratch instead of allocating."""
     lib = E.load_library()
     X = np.ascontiguousarray(X, dtype=np.float32)
     M = np.ascontiguousarray(M, dtype=np.float32)
@@ -116,10 +118,11 @@ def grad(X, M, Y, params):
     p = np.ascontiguousarray(params, dtype=np.float32)
     g = np.zeros(p.size, dtype=np.float32)
     loss = ctypes.c_double()
-    rc = lib.ccq_hip_mlp_grad(_fptr(X), _fptr(M), _fptr(Y), len(M), _fptr(p), p.size, _fptr(g),
-                              ctypes.byref(loss))
+    name = "ccq_hip_mlp_grad_noalloc" if noalloc else "ccq_hip_mlp_grad"
+    rc = getattr(lib, name)(_fptr(X), _fptr(M), _fptr(Y), len(M), _fptr(p), p.size, _fptr(g),
+                            ctypes.byref(loss))
     if rc != 0:
-        raise E.EngineError("ccq_hip_mlp_grad: " + lib.ccq_hip_mlp_last_error().decode())
+        raise E.EngineError(f"{name}: " + lib.ccq_hip_mlp_last_error().decode())
     return loss.value, g
 
 

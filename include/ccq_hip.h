@@ -202,6 +202,17 @@ int ccq_hip_mlp_train(const float* X, const float* M, const float* Y, int64_t n,
 /* Loss and gradient over all n samples (the training kernels, one step, no update). */
 int ccq_hip_mlp_grad(const float* X, const float* M, const float* Y, int64_t n,
                      const float* params, int64_t nparams, float* grad, double* loss);
+/* ccq_hip_mlp_grad without device allocation.  No counterpart in ccq: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as ccq_hip_mlp_grad; every device buffer
+ * is a static per-GPU array of about 9 MB held until the process ends, so
+ * n is limited to CCQ_HIP_MLP_NOALLOC_MAX_SAMPLES (larger n returns 2 and
+ * leaves the GPU untouched).  Calls that overlap on one GPU race (they share
+ * that array). */
+#define CCQ_HIP_MLP_NOALLOC_MAX_SAMPLES 8192
+int ccq_hip_mlp_grad_noalloc(const float* X, const float* M, const float* Y, int64_t n,
+                             const float* params, int64_t nparams, float* grad, double* loss);
 const char* ccq_hip_mlp_last_error(void);
 
 /* Arithmetic of the bulk contractions of later ccq_hip_update calls: 0 FP64
