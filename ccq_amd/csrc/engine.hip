@@ -142,6 +142,7 @@ struct Section {
   // order, one launch each.
   std::vector<std::vector<int>> launches;
   std::vector<std::string> launch_cls;
+  std::vector<int> launch_level;   // dependency level of each launch (non-decreasing)
   // chains: member op indices (program order), their orientation against the
   // first member (1: transposed) and the launch plan over the concatenated K
   std::vector<std::vector<int>> chains;
@@ -202,7 +203,17 @@ struct ccq_hip_ctx {
   std::vector<std::unique_ptr<DevBuf>> view_buf;
   struct Gather { int t; int64_t rows, cols, t_src_row, t_src_col, t_dst_row, t_dst_col; };
   std::vector<Gather> gathers;
-  DevBuf ws;                 // split-K workspace
+  DevBuf ws;                 // split-K workspace (main stream)
+  // launches of one dependency level run concurrently on up to nstreams
+  // streams (the main one and side[0..nstreams-2], default priority), each
+  // with its own split-K workspace; CCQ_HIP_STREAMS=1 runs everything on the
+  // main stream
+  static constexpr int kMaxStreams = 4;
+  int nstreams = kMaxStreams;
+  hipStream_t side[kMaxStreams - 1] = {};
+  DevBuf ws_side[kMaxStreams - 1];
+  hipEvent_t fork_ev = nullptr, join_ev[kMaxStreams - 1] = {};
+  double* cur_ws = nullptr;  // workspace of the stream being launched on
   int* itab = nullptr;
   double* dtab = nullptr;
   // program
@@ -239,6 +250,11 @@ struct ccq_hip_ctx {
     if (gather_tab) (void)hipFree(gather_tab);
     if (itab) (void)hipFree(itab);
     if (dtab) (void)hipFree(dtab);
+    for (auto& e : join_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    for (auto& st : side)
+      if (st) (void)hipStreamDestroy(st);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -1021,9 +1037,19 @@ struct ccq_hip_ctx {
     for (auto& sec : sections) {
       sec.launches.clear();
       sec.launch_cls.clear();
+      sec.launch_level.clear();
     }
     group_gemms(dump_schedule);
     ws.alloc(std::max<int64_t>(ws_need, 1));
+    cur_ws = ws.p;
+    if (nstreams > 1) {
+      HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+      for (int k = 0; k < nstreams - 1; ++k) {
+        HIPCHK(hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&join_ev[k], hipEventDisableTiming));
+        ws_side[k].alloc(std::max<int64_t>(ws_need, 1));
+      }
+    }
     if (lsq_n) lsq.alloc(lsq_n);
     HIPCHK(hipMalloc(&tables32, sizeof(int) * std::max<size_t>(host32.size(), 1)));
     if (!host32.empty())
@@ -1267,6 +1293,7 @@ struct ccq_hip_ctx {
         for (int i : idx) {
           sec.launches.push_back({i});
           sec.launch_cls.push_back(sec.ops[i].chain >= 0 ? chain_cls(sec, sec.ops[i].chain) : sec.ops[i].cls);
+          sec.launch_level.push_back((int)sec.launch_level.size());
         }
         continue;
       }
@@ -1376,9 +1403,11 @@ struct ccq_hip_ctx {
             sec.launches.push_back(it->second);
             sec.launch_cls.push_back(sec.ops[i].gemm ? group_cls(sec, it->second)
                                                      : "add:flat*" + std::to_string(it->second.size()));
+            sec.launch_level.push_back(lv);
           } else if (!grouped.count(i)) {
             sec.launches.push_back({i});
             sec.launch_cls.push_back(sec.ops[i].chain >= 0 ? chain_cls(sec, sec.ops[i].chain) : sec.ops[i].cls);
+            sec.launch_level.push_back(lv);
           }
         }
       }
@@ -1435,7 +1464,7 @@ struct ccq_hip_ctx {
     g.colC = tables32 + op.t_colC;
     g.alpha = op.coef;
     g.accumulate = op.assign ? 0 : 1;
-    g.ws = ws.p;
+    g.ws = cur_ws;
     g.prec = prec;
     if (op.fuse_a >= 0) {
       g.D = tensors[op.fuse_a].ptr;
@@ -1623,6 +1652,64 @@ struct ccq_hip_ctx {
 
   void sweep(int64_t nsec = -1) { run_program("sweep", nsec); }
 
+  std::string cur_sec;   // section being run (profile attribution)
+
+  // rough launch time (s) of launch k of a section: MFMA or HBM bound plus a
+  // fixed cost
+  double launch_cost(const Section& sec, size_t k) const {
+    double fl = 0, by = 0;
+    const auto& L = sec.launches[k];
+    std::vector<int> mem = L;
+    if (L.size() == 1 && sec.ops[L[0]].chain >= 0) mem = sec.chains[sec.ops[L[0]].chain];
+    for (int i : mem) fl += sec.ops[i].flops, by += sec.ops[i].bytes;
+    return std::max(fl / 40e12, by / 3e12) + 4e-6;
+  }
+
+  // The launches [a, b) of one dependency level (independent of each other:
+  // no two write the same tensor or read what another writes) on up to
+  // nstreams streams: longest first, each to the least loaded stream.  The
+  // side streams wait for the main stream's work so far and the main stream
+  // waits for theirs at the end, so the next level starts after all of this
+  // one (also under graph capture, where this becomes a fork / join).
+  template <typename F>
+  void run_level(const Section& sec, size_t a, size_t b, F&& run_launch) {
+    const int ns = std::min<int>(nstreams, (int)(b - a));
+    std::vector<size_t> order;
+    for (size_t k = a; k < b; ++k) order.push_back(k);
+    std::vector<double> cost(b - a);
+    for (size_t k = a; k < b; ++k) cost[k - a] = launch_cost(sec, k);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return cost[x - a] > cost[y - a]; });
+    std::vector<double> load(ns, 0.0);
+    std::vector<int> where(b - a, 0);
+    for (size_t k : order) {
+      const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      where[k - a] = s;
+      load[s] += cost[k - a];
+    }
+    HIPCHK(hipEventRecord(fork_ev, stream));
+    for (int s = 1; s < ns; ++s) HIPCHK(hipStreamWaitEvent(side[s - 1], fork_ev, 0));
+    hipStream_t main = stream;
+    for (size_t k = a; k < b; ++k) {
+      const int s = where[k - a];
+      stream = s ? side[s - 1] : main;
+      cur_ws = s ? ws_side[s - 1].p : ws.p;
+      try {
+        run_launch(k);
+      } catch (...) {
+        stream = main;
+        cur_ws = ws.p;
+        throw;
+      }
+    }
+    stream = main;
+    cur_ws = ws.p;
+    for (int s = 1; s < ns; ++s) {
+      HIPCHK(hipEventRecord(join_ev[s - 1], side[s - 1]));
+      HIPCHK(hipStreamWaitEvent(stream, join_ev[s - 1], 0));
+    }
+  }
+
   void run_program(const std::string& prog, int64_t nsec = -1) {
     if (!ints_ready) throw Fail("integrals not uploaded");
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1646,11 +1733,18 @@ struct ccq_hip_ctx {
       p.ms += ms;
       p.flops += fl;
       p.bytes += by;
+      // the same launch, summed per section (reported as "section:<name>")
+      auto& q = prof["section:" + cur_sec];
+      q.calls++;
+      q.ms += ms;
+      q.flops += fl;
+      q.bytes += by;
     };
     int64_t isec = 0;
     for (auto& sec : sections) {
       if (sec.prog != prog) continue;
       if (nsec >= 0 && isec++ >= nsec) break;
+      cur_sec = sec.name;
       if (sec.kind == Section::Cct3) {
         timed("cct3:" + sec.upd.block, 0, 0, [&] { run_cct3(sec.upd); });
       } else if (sec.kind == Section::Compute) {
@@ -1658,19 +1752,19 @@ struct ccq_hip_ctx {
           for (auto& op : sec.ops)
             if (!op.fused) timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
         } else {
-          for (size_t k = 0; k < sec.launches.size(); ++k) {
+          auto run_launch = [&](size_t k) {
             const auto& L = sec.launches[k];
             if (L.size() == 1 && sec.ops[L[0]].chain >= 0) {
               const int c = sec.ops[L[0]].chain;
               double fl = 0, by = 0;
               for (int i : sec.chains[c]) fl += sec.ops[i].flops, by += sec.ops[i].bytes;
               timed(sec.launch_cls[k], fl, by, [&] { run_chain(sec, c); });
-              continue;
+              return;
             }
             if (L.size() == 1) {
               const Op& op = sec.ops[L[0]];
               timed(op.cls, op.flops, op.bytes, [&] { run_op(op); });
-              continue;
+              return;
             }
             double fl = 0, by = 0;
             for (int i : L) fl += sec.ops[i].flops, by += sec.ops[i].bytes;
@@ -1678,6 +1772,16 @@ struct ccq_hip_ctx {
               timed(sec.launch_cls[k], fl, by, [&] { run_group(sec, L); });
             else
               timed(sec.launch_cls[k], fl, by, [&] { run_add_group(sec, L); });
+          };
+          for (size_t a = 0; a < sec.launches.size();) {
+            size_t b = a + 1;
+            while (b < sec.launches.size() && sec.launch_level[b] == sec.launch_level[a]) ++b;
+            if (b - a == 1 || nstreams == 1 || profile) {
+              for (size_t k = a; k < b; ++k) run_launch(k);
+            } else {
+              run_level(sec, a, b, run_launch);
+            }
+            a = b;
           }
         }
       } else if (sec.kind == Section::Update) {
@@ -2186,6 +2290,8 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (const char* g = std::getenv("CCQ_HIP_CHAIN")) ctx->chain = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_VIEWPERM")) ctx->view_perm = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_SCHEDULE")) ctx->dump_schedule = std::string(g) == "1";
+    if (const char* g = std::getenv("CCQ_HIP_STREAMS"))
+      ctx->nstreams = std::max(1, std::min(ccq_hip_ctx::kMaxStreams, std::atoi(g)));
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->layout();

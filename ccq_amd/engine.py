@@ -265,12 +265,16 @@ class Engine:
     def profile(self, enable: bool):
         self._check(self.lib.ccq_hip_profile(self.h, int(enable)), "profile")
 
-    def profile_report(self) -> dict:
+    def profile_report(self, sections: bool = False) -> dict:
+        """Per launch class {calls, ms, flops, bytes} of the profiled launches;
+        sections=True returns the per-section totals ("section:<name>") instead."""
         buf = ctypes.create_string_buffer(1 << 20)
         self._check(self.lib.ccq_hip_profile_report(self.h, buf, len(buf)), "profile_report")
         out = {}
         for line in buf.value.decode().splitlines():
             name, calls, ms, flops, nbytes = line.split()
+            if name.startswith("section:") != sections:
+                continue
             out[name] = dict(calls=int(calls), ms=float(ms), flops=float(flops), bytes=float(nbytes))
         return out
 

@@ -252,3 +252,50 @@ def test_fp32_grouped_and_chained_close_to_fp64(monkeypatch):
     diff = np.abs(out[1] - out[0]).max()
     assert diff > 1e-13, "the FP32 contractions did not run"
     assert diff < 1e-5 * step, (diff, step)
+
+
+@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 24, 5, 4), ("CCSDT", 12, 3, 2),
+                                           ("CCSDTQ", 8, 3, 2), ("CCSDt", 10, 3, 2)])
+def test_concurrent_levels_match_one_stream(method, n, o, ob, monkeypatch):
+    """The launches of a dependency level run on several streams (as a
+    fork / join inside the captured sweep graph); they are independent, so
+    three sweeps equal the one-stream order (CCQ_HIP_STREAMS=1) bit for bit,
+    eager and as graph replays."""
+    _say(f"case concurrent levels {method} n={n} o={o} ob={ob}")
+    c = Case(n, o, 2 * n, 53, occ_b=ob)
+    rng = np.random.default_rng(19)
+    t = osolver.zero_amplitudes("CCSDT", c.ca, c.cb, c.va, c.vb)
+    lvl = {"CCSD": 2, "CCSDT": 3, "CCSDTQ": 3, "CCSDt": 3}[method]
+    for k in t:
+        if osolver.BLOCKS[k][0] <= lvl:
+            t[k] = antisym_random(rng, k, c.ca, c.cb, c.va, c.vb, 0.03)
+    kw = dict(shift=0.03)
+    if method == "CCSDt":
+        from oracle import active as A
+        kw.update(act_occ=1, act_unocc=2)
+        act = A.ActiveSpace(0, c.oa, c.ob, c.n, 1, 2)
+        for k in ("t3aaa", "t3aab", "t3abb", "t3bbb"):
+            t[k][~A.p_mask(k, t[k].shape, act)] = 0.0
+        vec = A.tvec(t, act, c.ca, c.cb, c.va, c.vb)
+    else:
+        if method == "CCSD":
+            for k in ("t3aaa", "t3aab", "t3abb", "t3bbb"):
+                t[k] = np.zeros_like(t[k])
+        vec = E.join_tvec(t, c.ca, c.cb, c.va, c.vb)
+    runs = [("1", "0"), ("4", "0"), ("4", "1")]
+    engines = [_engine_env(c, method, monkeypatch, {"CCQ_HIP_STREAMS": s, "CCQ_HIP_GRAPH": g}, **kw)
+               for s, g in runs]
+    for eng in engines:
+        eng.set_t(vec)
+    for rep in range(3):
+        res = []
+        for eng in engines:
+            eng.update()
+            res.append((eng.energy(), eng.get_t(), eng.get_t4() if method == "CCSDTQ" else None))
+        for r in res[1:]:
+            assert r[0] == res[0][0], rep
+            np.testing.assert_array_equal(r[1], res[0][1])
+            if method == "CCSDTQ":
+                np.testing.assert_array_equal(r[2], res[0][2])
+    for eng in engines:
+        eng.close()
