@@ -207,6 +207,29 @@ def copy_probe(nbytes: int):
     log(f"copy probe: 3 x {nbytes} bytes")
 
 
+def sweeps_only(name, count, prec, rhf):
+    """Only `count` sweeps (graphed, as in the timed legs) from the state after
+    six ccq iterations: a short, representative program for profilers."""
+    method = synth.WORKLOADS[name]["method"]
+    s = synth.workload_system(name)
+    ints = build_integrals(s.h, s.e2int(), s.enuc, 0, s.nocc, s.nocc)
+    eng = E.Engine(0, s.nocc, s.nocc, s.norb, method, rhf=rhf, diis_space=5)
+    eng.set_ints(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb)
+    for it in range(1, 7):
+        eng.update()
+        eng.energy()
+        eng.diis_record(it)
+        eng.residuum(it)
+    eng.set_precision(prec)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(count):
+        eng.update()
+    eng.sync()
+    log(f"{name}: {count} sweeps (precision {prec}) in {(time.perf_counter() - t0) * 1e3:.2f} ms")
+    eng.close()
+
+
 def measure(name, steps, warmup, rhf, with_cpu, profile=True):
     diis = 5
     method = synth.WORKLOADS[name]["method"]
@@ -285,9 +308,15 @@ def main():
     ap.add_argument("--rhf", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (calibrates HBM counters)")
+    ap.add_argument("--sweeps-only", type=int, default=0, metavar="N",
+                    help="only run N graphed sweeps of --workload (for profilers)")
+    ap.add_argument("--precision", type=int, default=0, help="with --sweeps-only: 0 FP64, 1 FP32")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.sweeps_only:
+        sweeps_only(args.workload, args.sweeps_only, args.precision, args.rhf)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; sharding over several GPUs "

@@ -103,6 +103,9 @@ struct Op {
   // output (fused: this add is not launched; fuse_a: the GEMM's addend)
   bool fused = false;
   int fuse_a = -1;
+  // CCSDt: a triples residual product computed only on the P-space tuples
+  // its masked update reads (one rectangle of the split output)
+  bool restricted = false;
   double dcoef = 0.0;
   int64_t t_rowD = 0, t_colD = 0;
   // antisymmetric pairs packed to their canonical half: a summed pair (k),
@@ -835,6 +838,82 @@ struct ccq_hip_ctx {
     return out;
   }
 
+  // Active-space class of every entry of offsets(t, labels, subset, px, py)
+  // (same enumeration) for a triples residual block t: bit 0 an active
+  // occupied index, bit 1 an active alpha virtual, bit 2 an active beta
+  // virtual among the subset's indices.
+  std::vector<int> pclasses(const Tensor& t, const std::string& labels, const std::string& subset,
+                            int px = -1, int py = -1) const {
+    std::vector<int64_t> ext;
+    std::vector<int> kind;   // 0 occupied, 1 alpha virtual, 2 beta virtual
+    const int half = t.rank / 2;
+    const std::string spins = dimkey(t.base).substr(2);
+    for (char c : subset) {
+      const auto pos = labels.find(c);
+      if (pos == std::string::npos) throw Fail("label not in tensor");
+      ext.push_back(t.dims[pos]);
+      kind.push_back((int)pos < half ? 0 : (spins[pos % half] == 'b' ? 2 : 1));
+    }
+    if (subset.empty()) return std::vector<int>(1, 0);
+    int64_t total = 1;
+    for (auto e : ext) total *= e;
+    std::vector<int> out;
+    std::vector<int64_t> idx(ext.size(), 0);
+    for (int64_t n = 0; n < total; ++n) {
+      if (px < 0 || idx[px] < idx[py]) {
+        int cls = 0;
+        for (size_t k = 0; k < ext.size(); ++k) {
+          if (kind[k] == 0) cls |= idx[k] >= occ_start ? 1 : 0;
+          else if (idx[k] < vir_end[kind[k] - 1]) cls |= kind[k] == 1 ? 2 : 4;
+        }
+        out.push_back(cls);
+      }
+      for (int k = (int)ext.size() - 1; k >= 0; --k) {
+        if (++idx[k] < ext[k]) break;
+        idx[k] = 0;
+      }
+    }
+    if (out.empty()) out.push_back(0);
+    return out;
+  }
+
+  // The output of a triples residual product restricted to the tuples a
+  // masked update reads (an active occupied index, an active virtual of a
+  // spin in vneed, none of a spin in vforbid; jacobi_update's test): row and
+  // column index lists of the rectangles that cover exactly those elements
+  // (rows grouped by the set of column classes they need).
+  static std::vector<std::pair<std::vector<int64_t>, std::vector<int64_t>>> pspace_parts(
+      const std::vector<int>& rcl, const std::vector<int>& ccl, int vneed, int vforbid) {
+    auto pred = [&](int c) {
+      const bool va = c & 2, vb = c & 4;
+      const bool need = ((vneed & 1) && va) || ((vneed & 2) && vb);
+      const bool forbid = ((vforbid & 1) && va) || ((vforbid & 2) && vb);
+      return (c & 1) && need && !forbid;
+    };
+    std::map<int, std::vector<int64_t>> rows_by_set;
+    for (size_t m = 0; m < rcl.size(); ++m) {
+      int set = 0;
+      for (int c = 0; c < 8; ++c)
+        if (pred(rcl[m] | c)) set |= 1 << c;
+      if (set) rows_by_set[set].push_back((int64_t)m);
+    }
+    std::vector<std::pair<std::vector<int64_t>, std::vector<int64_t>>> parts;
+    for (auto& kv : rows_by_set) {
+      std::vector<int64_t> cols;
+      for (size_t n = 0; n < ccl.size(); ++n)
+        if ((kv.first >> ccl[n]) & 1) cols.push_back((int64_t)n);
+      if (!cols.empty()) parts.emplace_back(kv.second, cols);
+    }
+    return parts;
+  }
+
+  static std::vector<int64_t> pick(const std::vector<int64_t>& v, const std::vector<int64_t>& sel) {
+    std::vector<int64_t> out;
+    out.reserve(sel.size());
+    for (auto i : sel) out.push_back(v[(size_t)i]);
+    return out;
+  }
+
   static bool is_anti(const Tensor& t, const std::string& labels, char x, char y) {
     const auto p = labels.find(x), q = labels.find(y);
     if (p == std::string::npos || q == std::string::npos) return false;
@@ -900,6 +979,21 @@ struct ccq_hip_ctx {
     std::vector<int> ih;
     std::vector<double> dh;
     lsq_n = 0;
+    // CCSDt: a triples residual section feeds one masked update (the next
+    // update of its block); its products are computed on that update's tuples
+    std::map<const Section*, std::pair<int, int>> pmask;
+    if (active && pspace) {
+      for (size_t i = 0; i < sections.size(); ++i) {
+        const Section& sc = sections[i];
+        if (sc.kind != Section::Compute || sc.name.rfind("resid:t3", 0) != 0) continue;
+        const std::string blk = sc.name.substr(6);
+        for (size_t j = i + 1; j < sections.size(); ++j)
+          if (sections[j].kind == Section::Update && sections[j].upd.block == blk) {
+            if (sections[j].upd.masked) pmask[&sc] = {sections[j].upd.vneed, sections[j].upd.vforbid};
+            break;
+          }
+      }
+    }
     for (auto& sec : sections) {
       if (sec.kind == Section::Update || sec.kind == Section::Cct3) {
         sec.upd.dev_perm_off = (int)ih.size();
@@ -914,8 +1008,11 @@ struct ccq_hip_ctx {
         continue;
       }
       if (sec.kind == Section::Copy) continue;
-      for (auto& op : sec.ops) {
+      const auto pm_it = pmask.find(&sec);
+      std::vector<Op> planned;
+      for (Op& op : sec.ops) {
         const Tensor& C = tensors[op.out];
+        const bool restrict_out = pm_it != pmask.end() && C.kind == TKind::Resid && C.rank == 6;
         const Tensor& A = tensors[op.a];
         for (size_t i = 0; i < op.lc.size(); ++i) {
           // extents must agree label by label
@@ -970,12 +1067,34 @@ struct ccq_hip_ctx {
               pn = find_pair(ns, B, op.lb, nullptr, "");
             }
           }
-          auto rA = offsets(A, op.la, ms, pm.first, pm.second);
-          auto kA = offsets(A, op.la, ks, pk.first, pk.second);
-          auto kB = offsets(B, op.lb, ks, pk.first, pk.second);
-          auto cB = offsets(B, op.lb, ns, pn.first, pn.second);
-          auto rC = offsets(C, op.lc, ms, pm.first, pm.second);
-          auto cC = offsets(C, op.lc, ns, pn.first, pn.second);
+          const auto rA_all = offsets(A, op.la, ms, pm.first, pm.second);
+          const auto kA = offsets(A, op.la, ks, pk.first, pk.second);
+          const auto kB = offsets(B, op.lb, ks, pk.first, pk.second);
+          const auto cB_all = offsets(B, op.lb, ns, pn.first, pn.second);
+          const auto rC_all = offsets(C, op.lc, ms, pm.first, pm.second);
+          const auto cC_all = offsets(C, op.lc, ns, pn.first, pn.second);
+          std::vector<int64_t> rC2_all, cC2_all;
+          if (pm.first >= 0) rC2_all = offsets(C, op.lc, ms, pm.first, pm.second, true);
+          if (pn.first >= 0) cC2_all = offsets(C, op.lc, ns, pn.first, pn.second, true);
+          op.pack_k = pk.first >= 0;
+          if (pk.first >= 0) op.coef *= 2.0;
+          std::vector<std::pair<std::vector<int64_t>, std::vector<int64_t>>> parts;
+          if (restrict_out) {
+            parts = pspace_parts(pclasses(C, op.lc, ms, pm.first, pm.second),
+                                 pclasses(C, op.lc, ns, pn.first, pn.second), pm_it->second.first,
+                                 pm_it->second.second);
+          } else {
+            parts.emplace_back();
+          }
+          const Op base = op;
+          for (auto& part : parts) {
+          Op op = base;
+          const bool sub = restrict_out;
+          op.restricted = sub;
+          const auto rA = sub ? pick(rA_all, part.first) : rA_all;
+          const auto cB = sub ? pick(cB_all, part.second) : cB_all;
+          const auto rC = sub ? pick(rC_all, part.first) : rC_all;
+          const auto cC = sub ? pick(cC_all, part.second) : cC_all;
           op.M = (int64_t)rA.size();
           op.N = (int64_t)cB.size();
           op.K = (int64_t)kA.size();
@@ -985,10 +1104,8 @@ struct ccq_hip_ctx {
           op.t_colB = push32(cB);
           op.t_rowC = push32(rC);
           op.t_colC = push32(cC);
-          op.pack_k = pk.first >= 0;
-          if (pk.first >= 0) op.coef *= 2.0;
-          if (pm.first >= 0) op.t_rowC2 = push32(offsets(C, op.lc, ms, pm.first, pm.second, true));
-          if (pn.first >= 0) op.t_colC2 = push32(offsets(C, op.lc, ns, pn.first, pn.second, true));
+          if (pm.first >= 0) op.t_rowC2 = push32(sub ? pick(rC2_all, part.first) : rC2_all);
+          if (pn.first >= 0) op.t_colC2 = push32(sub ? pick(cC2_all, part.second) : cC2_all);
           ccqk::GemmArgs& g = op.plan;
           g.M = op.M;
           g.N = op.N;
@@ -1010,12 +1127,29 @@ struct ccq_hip_ctx {
                    (op.pack_k || op.t_rowC2 >= 0 || op.t_colC2 >= 0 ? "p" : "");
           op.flops = 2.0 * (double)op.M * (double)op.N * (double)(ks.empty() ? 1 : op.K);
           op.bytes = 8.0 * ((double)op.M * op.K + (double)op.K * op.N + 2.0 * op.M * op.N);
+          planned.push_back(op);
+          }
         } else {
           if (op.la.size() != op.lc.size()) throw Fail("add op must permute its input");
           ms = op.lc.substr(0, op.lc.size() - 1);
           ns = op.lc.substr(op.lc.size() - 1);
-          auto rA = offsets(A, op.la, ms), cA = offsets(A, op.la, ns);
-          auto rC = offsets(C, op.lc, ms), cC = offsets(C, op.lc, ns);
+          const auto rA_all = offsets(A, op.la, ms), cA_all = offsets(A, op.la, ns);
+          const auto rC_all = offsets(C, op.lc, ms), cC_all = offsets(C, op.lc, ns);
+          std::vector<std::pair<std::vector<int64_t>, std::vector<int64_t>>> parts;
+          if (restrict_out)
+            parts = pspace_parts(pclasses(C, op.lc, ms), pclasses(C, op.lc, ns), pm_it->second.first,
+                                 pm_it->second.second);
+          else
+            parts.emplace_back();
+          const Op base = op;
+          for (auto& part : parts) {
+          Op op = base;
+          const bool sub = restrict_out;
+          op.restricted = sub;
+          const auto rA = sub ? pick(rA_all, part.first) : rA_all;
+          const auto cA = sub ? pick(cA_all, part.second) : cA_all;
+          const auto rC = sub ? pick(rC_all, part.first) : rC_all;
+          const auto cC = sub ? pick(cC_all, part.second) : cC_all;
           op.M = (int64_t)rA.size();
           op.N = (int64_t)cA.size();
           // same layout on both sides: a flat (vectorised) axpy
@@ -1027,8 +1161,11 @@ struct ccq_hip_ctx {
           op.flops = (double)op.M * op.N;
           op.bytes = 8.0 * 3.0 * op.M * op.N;
           op.cls = op.flat ? "add:flat" : "add:perm";
+          planned.push_back(op);
+          }
         }
       }
+      sec.ops = std::move(planned);
     }
     fuse_adds(push32);
     // groups first; products left alone are chained where they can be
@@ -1076,7 +1213,7 @@ struct ccq_hip_ctx {
       auto& ops = sec.ops;
       for (size_t i = 0; i < ops.size(); ++i) {
         Op& ad = ops[i];
-        if (ad.gemm || ad.fused || ad.a == ad.out) continue;
+        if (ad.gemm || ad.fused || ad.a == ad.out || ad.restricted) continue;
         size_t j = i + 1;
         for (; j < ops.size(); ++j) {
           const Op& q = ops[j];
@@ -1084,7 +1221,7 @@ struct ccq_hip_ctx {
         }
         if (j == ops.size()) continue;
         Op& gm = ops[j];
-        if (!gm.gemm || gm.out != ad.out || gm.assign || gm.fuse_a >= 0 || gm.a == ad.out ||
+        if (!gm.gemm || gm.restricted || gm.out != ad.out || gm.assign || gm.fuse_a >= 0 || gm.a == ad.out ||
             gm.b == ad.out || gm.t_rowC2 >= 0 || gm.t_colC2 >= 0)
           continue;
         bool clobbered = false;
@@ -1260,6 +1397,8 @@ struct ccq_hip_ctx {
   // idle and that share a kernel instantiation run as one launch without
   // split-K.  CCQ_HIP_GROUP=0 keeps one launch per op in program order.
   bool group = true;
+  // CCSDt triples residuals on the P space only (CCQ_HIP_PSPACE=0: full blocks)
+  bool pspace = true;
   bool dump_schedule = false;   // CCQ_HIP_SCHEDULE=1: levels and launches to stderr
   static constexpr int64_t kGroupTiles = 512;   // below this a GEMM joins groups
   static constexpr int64_t kGroupMinK = 256;
@@ -2290,6 +2429,7 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (const char* g = std::getenv("CCQ_HIP_CHAIN")) ctx->chain = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_VIEWPERM")) ctx->view_perm = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_SCHEDULE")) ctx->dump_schedule = std::string(g) == "1";
+    if (const char* g = std::getenv("CCQ_HIP_PSPACE")) ctx->pspace = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_STREAMS"))
       ctx->nstreams = std::max(1, std::min(ccq_hip_ctx::kMaxStreams, std::atoi(g)));
     HIPCHK(hipSetDevice(ctx->device));
