@@ -207,6 +207,26 @@ def copy_probe(nbytes: int):
     log(f"copy probe: 3 x {nbytes} bytes")
 
 
+def mfma_ceiling(iters=2000):
+    """The attainable MFMA rate of this part (pure MFMA loops, 8 independent
+    accumulators per wave) for FP64, FP32 and BF16, at 1, 2 and 4 waves per
+    SIMD (256 CUs x 4 SIMDs); one JSON line per point."""
+    import ctypes
+    lib = E.load_library()
+    peaks = {0: ("f64 16x16x4", 78.6e12), 1: ("f32 16x16x4", 157.3e12), 2: ("bf16 16x16x32", 2.5e15)}
+    for kind, (name, peak) in peaks.items():
+        for blocks in (256, 512, 1024):
+            sec, fl = ctypes.c_double(), ctypes.c_double()
+            it = iters if kind < 2 else iters // 4
+            rc = lib.ccq_hip_mfma_ceiling(kind, blocks, it, ctypes.byref(sec), ctypes.byref(fl))
+            if rc != 0:
+                raise E.EngineError(f"ccq_hip_mfma_ceiling: HIP error {rc}")
+            rate = fl.value / sec.value
+            print(json.dumps({"mfma": name, "workgroups": blocks, "waves_per_simd": blocks // 256,
+                              "seconds": sec.value, "tflops": rate / 1e12, "peak_tflops": peak / 1e12,
+                              "frac": rate / peak}), flush=True)
+
+
 def sweeps_only(name, count, prec, rhf):
     """Only `count` sweeps (graphed, as in the timed legs) from the state after
     six ccq iterations: a short, representative program for profilers."""
@@ -311,7 +331,12 @@ def main():
     ap.add_argument("--sweeps-only", type=int, default=0, metavar="N",
                     help="only run N graphed sweeps of --workload (for profilers)")
     ap.add_argument("--precision", type=int, default=0, help="with --sweeps-only: 0 FP64, 1 FP32")
+    ap.add_argument("--mfma-ceiling", action="store_true",
+                    help="only measure the attainable MFMA rates (pure MFMA loops)")
     args = ap.parse_args()
+    if args.mfma_ceiling:
+        mfma_ceiling()
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return

@@ -2820,6 +2820,32 @@ int ccq_hip_profile_report(ccq_hip_ctx* ctx, char* buf, int64_t buflen) {
   });
 }
 
+int ccq_hip_mfma_ceiling(int64_t kind, int64_t blocks, int64_t iters, double* seconds, double* flops) {
+  if (kind < 0 || kind > 2 || blocks <= 0 || blocks > (1 << 20) || iters <= 0 || !seconds || !flops)
+    return (int)hipErrorInvalidValue;
+  double* out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&out, sizeof(double) * 256 * blocks);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double fl = 0.0;
+  if (e == hipSuccess) e = ccqk::launch_mfma_ceiling(nullptr, (int)kind, blocks, iters, out, &fl);
+  if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+  if (e == hipSuccess) e = ccqk::launch_mfma_ceiling(nullptr, (int)kind, blocks, iters, out, &fl);
+  if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess) {
+    *seconds = ms * 1e-3;
+    *flops = fl;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (out) (void)hipFree(out);
+  return (int)e;
+}
+
 int ccq_hip_sync(ccq_hip_ctx* ctx) {
   return guarded(ctx, [&] { HIPCHK(hipStreamSynchronize(ctx->stream)); });
 }

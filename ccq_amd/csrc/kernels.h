@@ -186,6 +186,9 @@ hipError_t launch_axpy_flat(hipStream_t s, int64_t n, const double* A, double* C
                             int accumulate);
 hipError_t launch_fill(hipStream_t s, double* x, int64_t n, double v);
 hipError_t launch_copy(hipStream_t s, double* y, const double* x, int64_t n);
+// MFMA issue-rate probe; out holds blocks * 256 doubles; *flops = work of the launch
+hipError_t launch_mfma_ceiling(hipStream_t s, int kind, int64_t blocks, int64_t iters, double* out,
+                               double* flops);
 // sumsq (optional): per-block partial sums of |G| R^2 over the canonical
 // tuples updated (the squared norm of the full-storage residual); one entry
 // per block of the launch, update_blocks(P) of them.

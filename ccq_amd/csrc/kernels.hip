@@ -1654,6 +1654,58 @@ hipError_t launch_axpy_group(hipStream_t s, const AddGroup& ag) {
   return hipGetLastError();
 }
 
+// MFMA issue-rate probe (the attainable matrix-core rate on this part, the
+// ceiling the contraction kernels are compared with): every wave runs `iters`
+// steps of 8 independent accumulator chains of one MFMA instruction; the sum
+// is stored so nothing is optimised away.  KIND 0: v_mfma_f64_16x16x4f64,
+// 1: v_mfma_f32_16x16x4f32, 2: v_mfma_f32_16x16x32_bf16.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+template <int KIND>
+__global__ __launch_bounds__(256) void mfma_ceiling(int64_t iters, double* __restrict__ out) {
+  const float x = 1.0f + 1e-3f * (float)(threadIdx.x & 63);
+  double s = 0.0;
+  if constexpr (KIND == 0) {
+    d4 acc[8] = {};
+    const double a = x, b = 2.0 - x;
+    for (int64_t i = 0; i < iters; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+    for (int j = 0; j < 8; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  } else if constexpr (KIND == 1) {
+    f4 acc[8] = {};
+    const float a = x, b = 2.0f - x;
+    for (int64_t i = 0; i < iters; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0);
+    for (int j = 0; j < 8; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  } else {
+    f4 acc[8] = {};
+    bf8 a, b;
+    for (int q = 0; q < 8; ++q) {
+      a[q] = (__bf16)x;
+      b[q] = (__bf16)(2.0f - x);
+    }
+    for (int64_t i = 0; i < iters; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+    for (int j = 0; j < 8; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  }
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+hipError_t launch_mfma_ceiling(hipStream_t s, int kind, int64_t blocks, int64_t iters, double* out,
+                               double* flops) {
+  const dim3 grid((unsigned)blocks), blk(256);
+  const double waves = (double)blocks * 4.0 * (double)iters * 8.0;
+  if (kind == 0) hipLaunchKernelGGL(mfma_ceiling<0>, grid, blk, 0, s, iters, out);
+  else if (kind == 1) hipLaunchKernelGGL(mfma_ceiling<1>, grid, blk, 0, s, iters, out);
+  else if (kind == 2) hipLaunchKernelGGL(mfma_ceiling<2>, grid, blk, 0, s, iters, out);
+  else return hipErrorInvalidValue;
+  // flops per wave instruction: 2*16*16*4 (FP64, FP32), 2*16*16*32 (BF16)
+  *flops = waves * (kind == 2 ? 16384.0 : 2048.0);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy(hipStream_t s, double* y, const double* x, int64_t n) {
   hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, y, x, n);
   return hipGetLastError();

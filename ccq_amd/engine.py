@@ -99,6 +99,9 @@ EXPORTS = {
     "ccq_hip_profile_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "ccq_hip_debug_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_debug_sweep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+    "ccq_hip_mfma_ceiling": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double)]),
     "ccq_hip_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "ccq_hip_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "ccq_hip_destroy": (ctypes.c_int, [ctypes.c_void_p]),

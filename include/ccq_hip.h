@@ -238,6 +238,12 @@ int ccq_hip_debug_sweep(ccq_hip_ctx* ctx, int64_t nsections);
 
 /* Block until the context's stream is idle. */
 int ccq_hip_sync(ccq_hip_ctx* ctx);
+/* Attainable matrix-core rate (no ccq counterpart; the ceiling the contraction
+ * kernels are measured against): `blocks` workgroups of 4 waves each issue
+ * `iters` x 8 independent MFMAs (kind 0 v_mfma_f64_16x16x4f64, 1
+ * v_mfma_f32_16x16x4f32, 2 v_mfma_f32_16x16x32_bf16) on the current device;
+ * returns the second launch's time and its flops.  0 or the HIP error code. */
+int ccq_hip_mfma_ceiling(int64_t kind, int64_t blocks, int64_t iters, double* seconds, double* flops);
 
 const char* ccq_hip_last_error(const ccq_hip_ctx* ctx);
 int ccq_hip_destroy(ccq_hip_ctx* ctx);
