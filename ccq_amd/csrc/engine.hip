@@ -49,7 +49,7 @@ struct Fail : std::runtime_error {
       throw Fail(std::string(#x) + ": " + hipGetErrorString(e_));                  \
   } while (0)
 
-constexpr int kPartialBlocks = 240;
+constexpr int kPartialBlocks = 1024;
 constexpr int64_t kWsCap = 16 << 20;   // split-K workspace bound (doubles)
 
 // ---------------------------------------------------------------------------
@@ -132,6 +132,9 @@ struct UpdateStep {
   int vneed = 0, vforbid = 0;
   bool left = false;         // left-CC step: H-bar denominators, |residual|^2 partials
   int sq_off = 0, sq_n = 0;  // this step's slice of the partial-sum buffer
+  // canonical occupied / virtual halves of the block's tuples (int table offsets)
+  int t_coc = -1, t_cvc = -1;
+  int64_t n_oc = 0, n_vc = 0;
   int m3 = -1;               // triples correction: the M3 accumulator (resid = L3)
 };
 
@@ -1000,6 +1003,37 @@ struct ccq_hip_ctx {
         ih.insert(ih.end(), sec.upd.perms.begin(), sec.upd.perms.end());
         sec.upd.dev_sign_off = (int)dh.size();
         dh.insert(dh.end(), sec.upd.signs.begin(), sec.upd.signs.end());
+        if (sec.kind == Section::Update) {
+          // canonical halves: every tuple of the half in index order, kept
+          // when ascending inside each antisymmetric group
+          const ccqk::UpdateParams P0 = update_params(sec.upd);
+          const int half = P0.rank / 2;
+          for (int h = 0; h < 2; ++h) {
+            const int d0 = h * half;
+            const int64_t off = (int64_t)ih.size();
+            int64_t count = 0;
+            std::vector<int64_t> idx(half, 0);
+            int64_t total = 1;
+            for (int d = 0; d < half; ++d) total *= P0.dims[d0 + d];
+            for (int64_t e = 0; e < total; ++e) {
+              bool canon = true;
+              for (int d = 0; d + 1 < half; ++d)
+                if (P0.group[d0 + d] >= 0 && P0.group[d0 + d] == P0.group[d0 + d + 1] && idx[d] >= idx[d + 1])
+                  canon = false;
+              if (canon) {
+                for (int d = 0; d < half; ++d) ih.push_back((int)idx[d]);
+                ++count;
+              }
+              for (int d = half - 1; d >= 0; --d) {
+                if (++idx[d] < P0.dims[d0 + d]) break;
+                idx[d] = 0;
+              }
+            }
+            if (off > 0x7fffffffLL || (int64_t)ih.size() > 0x7fffffffLL) throw Fail("update table too large");
+            (h == 0 ? sec.upd.t_coc : sec.upd.t_cvc) = (int)off;
+            (h == 0 ? sec.upd.n_oc : sec.upd.n_vc) = count;
+          }
+        }
         if (sec.kind == Section::Update && sec.upd.left) {
           sec.upd.sq_off = (int)lsq_n;
           sec.upd.sq_n = ccqk::update_blocks(update_params(sec.upd));
@@ -1705,6 +1739,14 @@ struct ccq_hip_ctx {
           P.group[h * half + k + 1] = h * 16 + (spins[k] == 'b');
         }
     P.shift = shift;
+    if (u.t_coc >= 0) {
+      P.total = u.n_oc * u.n_vc;
+      P.n_vc = std::max<int64_t>(u.n_vc, 1);
+      if (itab) {
+        P.coc = itab + u.t_coc;
+        P.cvc = itab + u.t_cvc;
+      }
+    }
     if (u.masked) {
       P.mask = 1;
       P.vneed = u.vneed;
@@ -1718,6 +1760,7 @@ struct ccq_hip_ctx {
 
   void run_update(const UpdateStep& u) {
     ccqk::UpdateParams P = update_params(u);
+    if (!P.coc || !P.cvc) throw Fail("update step " + u.block + " has no canonical tuple tables");
     const Tensor& R = tensors[u.resid];
     if (u.left) {
       // left_jacobi_update (src/update_lcc.f90:208-280): L -= LH / (Hbar_aa - Hbar_ii - E_cor + shift)
@@ -1952,11 +1995,11 @@ struct ccq_hip_ctx {
     // src/energy.f90:138-270, same eight pieces
     double* t1a = tvec.p + blk_off["t1a"];
     double* t1b = tvec.p + blk_off["t1b"];
-    double total = 0.0;
+    // each piece's block partials in its own slice, one copy to the host,
+    // then the pieces summed in order
     HIPCHK(ccqk::launch_dot_strided(stream, oa, va, Fa.p + oa, nc, t1a, partial.p, kPartialBlocks));
-    total += reduce_partials(kPartialBlocks);
-    HIPCHK(ccqk::launch_dot_strided(stream, ob, vb, Fb.p + ob, nc, t1b, partial.p, kPartialBlocks));
-    total += reduce_partials(kPartialBlocks);
+    HIPCHK(ccqk::launch_dot_strided(stream, ob, vb, Fb.p + ob, nc, t1b, partial.p + kPartialBlocks,
+                                    kPartialBlocks));
     struct Case {
       const double* V;
       int64_t o1, o2;
@@ -1968,6 +2011,7 @@ struct ccq_hip_ctx {
     Case cases[3] = {{Vaa.p, oa, oa, "t2aa", t1a, t1a, 0.25, 0.5},
                      {Vab.p, oa, ob, "t2ab", t1a, t1b, 1.0, 1.0},
                      {vbb_ptr, ob, ob, "t2bb", t1b, t1b, 0.25, 0.5}};
+    int piece = 2;
     for (auto& c : cases) {
       ccqk::EnergyParams P{};
       P.V = c.V;
@@ -1986,8 +2030,18 @@ struct ccq_hip_ctx {
       P.v2 = nc - c.o2;
       P.w2 = c.w2;
       P.w11 = c.w11;
-      HIPCHK(ccqk::launch_energy_pairs(stream, P, partial.p, kPartialBlocks));
-      total += reduce_partials(kPartialBlocks);
+      HIPCHK(ccqk::launch_energy_pairs(stream, P, partial.p + (int64_t)piece++ * kPartialBlocks,
+                                       kPartialBlocks));
+    }
+    host_partial.resize((size_t)piece * kPartialBlocks);
+    HIPCHK(hipMemcpyAsync(host_partial.data(), partial.p, sizeof(double) * piece * kPartialBlocks,
+                          hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    double total = 0.0;
+    for (int q = 0; q < piece; ++q) {
+      double sp = 0.0;
+      for (int b = 0; b < kPartialBlocks; ++b) sp += host_partial[(size_t)q * kPartialBlocks + b];
+      total += sp;
     }
     return total;
   }

@@ -76,6 +76,12 @@ struct UpdateParams {
   int vneed = 0, vforbid = 0;
   int64_t occ_start = 0;
   int64_t vir_end[2] = {0, 0};
+  // canonical tuples (indices ascending inside every antisymmetric group):
+  // the occupied halves coc[n_oc][rank/2] and the virtual halves
+  // cvc[n_vc][rank/2]; the kernel runs over total = n_oc * n_vc of them
+  const int* coc = nullptr;
+  const int* cvc = nullptr;
+  int64_t n_vc = 1;
 };
 
 // CC(t;3) triples correction of one spin case (crcc23, src/mm_correct.f90:11-343).

@@ -866,19 +866,19 @@ __global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __r
                                                      const double* __restrict__ fdiag_b,
                                                      double* __restrict__ sumsq) {
   const int rank = P.rank;
+  const int half = rank / 2;
   double sq = 0.0;
+  // one thread per canonical tuple (occupied half oc, virtual half vc; the
+  // virtual half runs fastest, so the identity image is read and written
+  // along its last index)
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < P.total;
        e += (int64_t)gridDim.x * blockDim.x) {
     int64_t idx[8];
-    int64_t r = e;
-    for (int d = rank - 1; d >= 0; --d) {
-      idx[d] = r % P.dims[d];
-      r /= P.dims[d];
+    const int64_t oc = e / P.n_vc, vc = e - oc * P.n_vc;
+    for (int d = 0; d < half; ++d) {
+      idx[d] = P.coc[oc * half + d];
+      idx[half + d] = P.cvc[vc * half + d];
     }
-    bool canon = true;
-    for (int d = 0; d + 1 < rank; ++d)
-      if (P.group[d] >= 0 && P.group[d] == P.group[d + 1] && idx[d] >= idx[d + 1]) canon = false;
-    if (!canon) continue;
     if (P.mask) {
       bool occ_act = false, vir_a = false, vir_b = false;
       for (int d = 0; d < rank; ++d) {
@@ -910,6 +910,81 @@ __global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __r
     for (int d = 0; d < rank; ++d) off0 += idx[d] * P.strides[d];
     const double tn = t[off0] - R / D;
     for (int g = 0; g < P.nimg; ++g) {
+      int64_t off = 0;
+      for (int d = 0; d < rank; ++d) off += idx[d] * P.strides[perms[g * 8 + d]];
+      t[off] = signs[g] * tn;
+    }
+  }
+  if (SUMSQ) {
+    __shared__ double red[256];
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) sumsq[blockIdx.x] = red[0];
+  }
+}
+
+// The same step with one wave per canonical tuple, for blocks with many
+// images (a T4 block of one spin has 576): lane l gathers and scatters images
+// l, l + 64, ...; the image sum is a fixed butterfly over the wave.
+template <bool SUMSQ>
+__global__ __launch_bounds__(256) void jacobi_update_wave(UpdateParams P, double* __restrict__ t,
+                                                          const double* __restrict__ U,
+                                                          const int* __restrict__ perms,
+                                                          const double* __restrict__ signs,
+                                                          const double* __restrict__ fdiag_a,
+                                                          const double* __restrict__ fdiag_b,
+                                                          double* __restrict__ sumsq) {
+  const int rank = P.rank;
+  const int half = rank / 2;
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  double sq = 0.0;
+  for (int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; e < P.total; e += nwaves) {
+    int64_t idx[8];
+    const int64_t oc = e / P.n_vc, vc = e - oc * P.n_vc;
+    for (int d = 0; d < half; ++d) {
+      idx[d] = P.coc[oc * half + d];
+      idx[half + d] = P.cvc[vc * half + d];
+    }
+    if (P.mask) {
+      bool occ_act = false, vir_a = false, vir_b = false;
+      for (int d = 0; d < rank; ++d) {
+        if (!P.isvir[d]) {
+          occ_act |= idx[d] >= P.occ_start;
+        } else if (idx[d] < P.vir_end[P.spin[d]]) {
+          if (P.spin[d]) vir_b = true; else vir_a = true;
+        }
+      }
+      const bool need = ((P.vneed & 1) && vir_a) || ((P.vneed & 2) && vir_b);
+      const bool forbid = ((P.vforbid & 1) && vir_a) || ((P.vforbid & 2) && vir_b);
+      if (!occ_act || !need || forbid) continue;
+    }
+    double R = 0.0;
+    for (int g = lane; g < P.nimg; g += 64) {
+      int64_t off = 0;
+      for (int d = 0; d < rank; ++d) off += idx[d] * P.strides[perms[g * 8 + d]];
+      R += signs[g] * U[off];
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) R += __shfl_xor(R, m);
+    R /= (double)P.nimg;
+    if (SUMSQ && lane == 0) sq += (double)P.nimg * R * R;
+    double D = 0.0;
+    for (int d = 0; d < rank; ++d)
+      if (P.isvir[d]) D += (P.spin[d] ? fdiag_b : fdiag_a)[P.occ[d] + idx[d]];
+    for (int d = 0; d < rank; ++d)
+      if (!P.isvir[d]) D -= (P.spin[d] ? fdiag_b : fdiag_a)[idx[d]];
+    D += P.shift;
+    int64_t off0 = 0;
+    for (int d = 0; d < rank; ++d) off0 += idx[d] * P.strides[d];
+    // every lane has read t[off0] before any lane stores an image (the store
+    // depends on tn)
+    const double tn = t[off0] - R / D;
+    for (int g = lane; g < P.nimg; g += 64) {
       int64_t off = 0;
       for (int d = 0; d < rank; ++d) off += idx[d] * P.strides[perms[g * 8 + d]];
       t[off] = signs[g] * tn;
@@ -1716,11 +1791,26 @@ hipError_t launch_fill(hipStream_t s, double* x, int64_t n, double v) {
   return hipGetLastError();
 }
 
-int update_blocks(const UpdateParams& P) { return grid_for(P.total, 256); }
+// blocks with at least kWaveImages images run one wave per canonical tuple
+constexpr int kWaveImages = 128;
+int update_blocks(const UpdateParams& P) {
+  return P.nimg >= kWaveImages ? grid_for(P.total * 64, 256, 16384) : grid_for(P.total, 256);
+}
 
 hipError_t launch_update(hipStream_t s, const UpdateParams& P, double* t, const double* U,
                          const int* perms, const double* signs, const double* fa,
                          const double* fb, double* sumsq) {
+  if (!P.coc || !P.cvc || P.n_vc <= 0) return hipErrorInvalidValue;
+  if (P.total <= 0) return hipSuccess;
+  if (P.nimg >= kWaveImages) {
+    if (sumsq)
+      hipLaunchKernelGGL(jacobi_update_wave<true>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U,
+                         perms, signs, fa, fb, sumsq);
+    else
+      hipLaunchKernelGGL(jacobi_update_wave<false>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U,
+                         perms, signs, fa, fb, sumsq);
+    return hipGetLastError();
+  }
   if (sumsq)
     hipLaunchKernelGGL(jacobi_update<true>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U, perms,
                        signs, fa, fb, sumsq);
