@@ -106,6 +106,12 @@ struct Op {
   // CCSDt: a triples residual product computed only on the P-space tuples
   // its masked update reads (one rectangle of the split output)
   bool restricted = false;
+  // one of the parts an op was split into (P-space rectangles, irrep
+  // blocks): parts of one op write disjoint elements, so they may share a
+  // dependency level; gplan is the tile plan all parts share in a grouped launch
+  int part_of = -1;
+  bool has_gplan = false;
+  ccqk::GemmArgs gplan{};
   double dcoef = 0.0;
   int64_t t_rowD = 0, t_colD = 0;
   // antisymmetric pairs packed to their canonical half: a summed pair (k),
@@ -530,6 +536,7 @@ struct ccq_hip_ctx {
             std::string s;
             ls >> s;
             t.dims[i] = dimsym(s);
+            t.spec[i] = s;
           }
         } else if (kind == "view") {
           t.kind = TKind::View;
@@ -641,6 +648,8 @@ struct ccq_hip_ctx {
         inter_buf.emplace_back(new DevBuf);
         inter_buf.back()->alloc(std::max<int64_t>(s, 1));
         t.ptr = inter_buf.back()->p;
+        // elements no product writes (symmetry-forbidden blocks) stay zero
+        HIPCHK(hipMemset(t.ptr, 0, sizeof(double) * std::max<int64_t>(s, 1)));
       } else if (t.kind == TKind::Amp || t.kind == TKind::Resid) {
         int64_t s = 1;
         for (int i = t.rank - 1; i >= 0; --i) {
@@ -659,6 +668,7 @@ struct ccq_hip_ctx {
           if (!rb) {
             rb.reset(new DevBuf);
             rb->alloc(s);
+            HIPCHK(hipMemset(rb->p, 0, sizeof(double) * std::max<int64_t>(s, 1)));
           }
           t.ptr = rb->p;
         }
@@ -841,6 +851,53 @@ struct ccq_hip_ctx {
     return out;
   }
 
+  // (virtual?, spin) of position pos of tensor t
+  void pos_space(const Tensor& t, int pos, bool& vir, int& spin) const {
+    if (t.kind == TKind::View || t.kind == TKind::Inter) {
+      vir = t.spec[pos][0] == 'v';
+      spin = t.spec[pos][1] == 'b' ? 1 : 0;
+      return;
+    }
+    const int half = t.rank / 2;
+    const std::string spins = dimkey(t.base).substr(2);
+    vir = pos >= half;
+    spin = spins[pos % half] == 'b' ? 1 : 0;
+  }
+
+  // Irrep (XOR of the orbital irreps) of every entry of
+  // offsets(t, labels, subset, px, py), same enumeration
+  std::vector<int> irreps_of(const Tensor& t, const std::string& labels, const std::string& subset,
+                             int px = -1, int py = -1) const {
+    std::vector<int64_t> ext, base;
+    for (char c : subset) {
+      const auto pos = labels.find(c);
+      if (pos == std::string::npos) throw Fail("label not in tensor");
+      bool vir;
+      int spin;
+      pos_space(t, (int)pos, vir, spin);
+      ext.push_back(t.dims[pos]);
+      base.push_back(vir ? (spin ? ob : oa) : 0);
+    }
+    if (subset.empty()) return std::vector<int>(1, 0);
+    int64_t total = 1;
+    for (auto e : ext) total *= e;
+    std::vector<int> out;
+    std::vector<int64_t> idx(ext.size(), 0);
+    for (int64_t n = 0; n < total; ++n) {
+      if (px < 0 || idx[px] < idx[py]) {
+        int g = 0;
+        for (size_t k = 0; k < ext.size(); ++k) g ^= orb_irrep[(size_t)(base[k] + idx[k])];
+        out.push_back(g);
+      }
+      for (int k = (int)ext.size() - 1; k >= 0; --k) {
+        if (++idx[k] < ext[k]) break;
+        idx[k] = 0;
+      }
+    }
+    if (out.empty()) out.push_back(0);
+    return out;
+  }
+
   // Active-space class of every entry of offsets(t, labels, subset, px, py)
   // (same enumeration) for a triples residual block t: bit 0 an active
   // occupied index, bit 1 an active alpha virtual, bit 2 an active beta
@@ -908,6 +965,50 @@ struct ccq_hip_ctx {
       if (!cols.empty()) parts.emplace_back(kv.second, cols);
     }
     return parts;
+  }
+
+  // A piece of a product: its rows, columns and summed entries (empty ks:
+  // all of them)
+  struct Part {
+    std::vector<int64_t> rows, cols, ks;
+    bool no_k = false;   // a block no summed entry reaches (an assigning product writes zeros)
+  };
+
+  static std::vector<int64_t> iota64(int64_t n) {
+    std::vector<int64_t> v((size_t)std::max<int64_t>(n, 0));
+    for (int64_t i = 0; i < n; ++i) v[(size_t)i] = i;
+    return v;
+  }
+
+  // Irrep blocks of each part: for every irrep g the rows, columns and
+  // summed entries of irrep g (a totally symmetric A[m,k] vanishes unless
+  // irrep(m) = irrep(k), and B[k,n] unless irrep(k) = irrep(n)); blocks
+  // without rows or columns are dropped, and so are blocks of a product
+  // without summed entries unless the product assigns its output (then the
+  // block is kept, no_k, and written with zeros)
+  static std::vector<Part> irrep_split(const std::vector<Part>& parts, const std::vector<int>& rir,
+                                       const std::vector<int>& cir, const std::vector<int>* kir,
+                                       bool assign) {
+    std::vector<Part> out;
+    for (const Part& p : parts) {
+      for (int g = 0; g < 8; ++g) {
+        Part q;
+        for (auto m : p.rows)
+          if (rir[(size_t)m] == g) q.rows.push_back(m);
+        for (auto n : p.cols)
+          if (cir[(size_t)n] == g) q.cols.push_back(n);
+        if (kir) {
+          for (size_t k = 0; k < kir->size(); ++k)
+            if ((*kir)[k] == g) q.ks.push_back((int64_t)k);
+          if (q.ks.empty()) {
+            if (!assign) continue;
+            q.no_k = true;
+          }
+        }
+        if (!q.rows.empty() && !q.cols.empty()) out.push_back(std::move(q));
+      }
+    }
+    return out;
   }
 
   static std::vector<int64_t> pick(const std::vector<int64_t>& v, const std::vector<int64_t>& sel) {
@@ -1112,34 +1213,46 @@ struct ccq_hip_ctx {
           if (pn.first >= 0) cC2_all = offsets(C, op.lc, ns, pn.first, pn.second, true);
           op.pack_k = pk.first >= 0;
           if (pk.first >= 0) op.coef *= 2.0;
-          std::vector<std::pair<std::vector<int64_t>, std::vector<int64_t>>> parts;
+          std::vector<Part> parts(1);
+          parts[0].rows = iota64((int64_t)rA_all.size());
+          parts[0].cols = iota64((int64_t)cB_all.size());
           if (restrict_out) {
-            parts = pspace_parts(pclasses(C, op.lc, ms, pm.first, pm.second),
-                                 pclasses(C, op.lc, ns, pn.first, pn.second), pm_it->second.first,
-                                 pm_it->second.second);
-          } else {
-            parts.emplace_back();
+            parts.clear();
+            for (auto& rc : pspace_parts(pclasses(C, op.lc, ms, pm.first, pm.second),
+                                         pclasses(C, op.lc, ns, pn.first, pn.second),
+                                         pm_it->second.first, pm_it->second.second))
+              parts.push_back(Part{rc.first, rc.second, {}});
           }
+          if (use_sym) {
+            const auto kir = irreps_of(A, op.la, ks, pk.first, pk.second);
+            parts = irrep_split(parts, irreps_of(C, op.lc, ms, pm.first, pm.second),
+                                irreps_of(C, op.lc, ns, pn.first, pn.second), &kir, op.assign);
+          }
+          const bool sub = restrict_out || use_sym;
+          const int pid = sub && parts.size() > 1 ? part_count++ : -1;
           const Op base = op;
+          const size_t first_part = planned.size();
           for (auto& part : parts) {
           Op op = base;
-          const bool sub = restrict_out;
           op.restricted = sub;
-          const auto rA = sub ? pick(rA_all, part.first) : rA_all;
-          const auto cB = sub ? pick(cB_all, part.second) : cB_all;
-          const auto rC = sub ? pick(rC_all, part.first) : rC_all;
-          const auto cC = sub ? pick(cC_all, part.second) : cC_all;
+          op.part_of = pid;
+          const auto rA = sub ? pick(rA_all, part.rows) : rA_all;
+          const auto cB = sub ? pick(cB_all, part.cols) : cB_all;
+          const auto rC = sub ? pick(rC_all, part.rows) : rC_all;
+          const auto cC = sub ? pick(cC_all, part.cols) : cC_all;
+          const auto kAp = part.no_k ? std::vector<int64_t>() : part.ks.empty() ? kA : pick(kA, part.ks);
+          const auto kBp = part.no_k ? std::vector<int64_t>() : part.ks.empty() ? kB : pick(kB, part.ks);
           op.M = (int64_t)rA.size();
           op.N = (int64_t)cB.size();
-          op.K = (int64_t)kA.size();
+          op.K = (int64_t)kAp.size();
           op.t_rowA = push32(rA);
-          op.t_kA = push32(kA);
-          op.t_kB = push32(kB);
+          op.t_kA = push32(kAp);
+          op.t_kB = push32(kBp);
           op.t_colB = push32(cB);
           op.t_rowC = push32(rC);
           op.t_colC = push32(cC);
-          if (pm.first >= 0) op.t_rowC2 = push32(sub ? pick(rC2_all, part.first) : rC2_all);
-          if (pn.first >= 0) op.t_colC2 = push32(sub ? pick(cC2_all, part.second) : cC2_all);
+          if (pm.first >= 0) op.t_rowC2 = push32(sub ? pick(rC2_all, part.rows) : rC2_all);
+          if (pn.first >= 0) op.t_colC2 = push32(sub ? pick(cC2_all, part.cols) : cC2_all);
           ccqk::GemmArgs& g = op.plan;
           g.M = op.M;
           g.N = op.N;
@@ -1148,13 +1261,25 @@ struct ccq_hip_ctx {
                       : (!ms.empty() && stride_of(A, op.la, ms.back()) == 1 ? 0 : 1));
           g.b_kfast = ks.empty() ? 1 : (stride_of(B, op.lb, ks.back()) == 1 ? 1
                       : (!ns.empty() && stride_of(B, op.lb, ns.back()) == 1 ? 0 : 1));
-          g.kA_aff = affine_stride(kA);
-          g.kB_aff = affine_stride(kB);
+          g.kA_aff = affine_stride(kAp);
+          g.kB_aff = affine_stride(kBp);
           // C contiguous along m (n): its fastest m (n) label has stride 1,
           // so consecutive rows (columns) are adjacent in runs of that extent
           g.c_mfast = !ms.empty() && stride_of(C, op.lc, ms.back()) == 1 ? 1 : 0;
           g.c_nfast = !ns.empty() && stride_of(C, op.lc, ns.back()) == 1 ? 1 : 0;
-          ccqk::gemm_plan(g, kWsCap);
+          if (op.K == 0) {
+            // an irrep block no summed entry reaches: the tile kernel with no
+            // K slab stores zeros (the product assigns its output)
+            g.kind = 0;
+            g.trans = 0;
+            g.bm = g.bn = 32;
+            g.bk = 32;
+            g.ksplit = 1;
+            g.nsplit = 1;
+            g.kchunk = 0;
+          } else {
+            ccqk::gemm_plan(g, kWsCap);
+          }
           ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(g));
           op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" +
                    std::to_string(op.K) + "/" + plan_tag(g) +
@@ -1163,27 +1288,58 @@ struct ccq_hip_ctx {
           op.bytes = 8.0 * ((double)op.M * op.K + (double)op.K * op.N + 2.0 * op.M * op.N);
           planned.push_back(op);
           }
+          if (pid >= 0) {
+            // the parts run as grouped launches: one tile shape for all
+            int64_t mmax = 0, nmax = 0;
+            for (size_t q = first_part; q < planned.size(); ++q) {
+              mmax = std::max(mmax, planned[q].M);
+              nmax = std::max(nmax, planned[q].N);
+            }
+            for (size_t q = first_part; q < planned.size(); ++q) {
+              // a long summed range stays with its own plan (split-K, or the
+              // register-direct kernel): a grouped member runs its whole K
+              if (planned[q].K > kPartGroupMaxK) continue;
+              ccqk::GemmArgs gp = planned[q].plan;
+              gp.kind = 0;
+              gp.trans = 0;
+              gp.ksplit = 1;
+              gp.bm = mmax >= 96 ? 64 : 32;
+              gp.bn = nmax >= 96 ? 64 : 32;
+              gp.bk = 32;
+              gp.nsplit = 1;
+              gp.kchunk = gp.K;
+              planned[q].gplan = gp;
+              planned[q].has_gplan = true;
+            }
+          }
         } else {
           if (op.la.size() != op.lc.size()) throw Fail("add op must permute its input");
           ms = op.lc.substr(0, op.lc.size() - 1);
           ns = op.lc.substr(op.lc.size() - 1);
           const auto rA_all = offsets(A, op.la, ms), cA_all = offsets(A, op.la, ns);
           const auto rC_all = offsets(C, op.lc, ms), cC_all = offsets(C, op.lc, ns);
-          std::vector<std::pair<std::vector<int64_t>, std::vector<int64_t>>> parts;
-          if (restrict_out)
-            parts = pspace_parts(pclasses(C, op.lc, ms), pclasses(C, op.lc, ns), pm_it->second.first,
-                                 pm_it->second.second);
-          else
-            parts.emplace_back();
+          std::vector<Part> parts(1);
+          parts[0].rows = iota64((int64_t)rA_all.size());
+          parts[0].cols = iota64((int64_t)cA_all.size());
+          if (restrict_out) {
+            parts.clear();
+            for (auto& rc : pspace_parts(pclasses(C, op.lc, ms), pclasses(C, op.lc, ns),
+                                         pm_it->second.first, pm_it->second.second))
+              parts.push_back(Part{rc.first, rc.second, {}});
+          }
+          if (use_sym)
+            parts = irrep_split(parts, irreps_of(C, op.lc, ms), irreps_of(C, op.lc, ns), nullptr, op.assign);
+          const bool sub = restrict_out || use_sym;
+          const int pid = sub && parts.size() > 1 ? part_count++ : -1;
           const Op base = op;
           for (auto& part : parts) {
           Op op = base;
-          const bool sub = restrict_out;
           op.restricted = sub;
-          const auto rA = sub ? pick(rA_all, part.first) : rA_all;
-          const auto cA = sub ? pick(cA_all, part.second) : cA_all;
-          const auto rC = sub ? pick(rC_all, part.first) : rC_all;
-          const auto cC = sub ? pick(cC_all, part.second) : cC_all;
+          op.part_of = pid;
+          const auto rA = sub ? pick(rA_all, part.rows) : rA_all;
+          const auto cA = sub ? pick(cA_all, part.cols) : cA_all;
+          const auto rC = sub ? pick(rC_all, part.rows) : rC_all;
+          const auto cC = sub ? pick(cC_all, part.cols) : cC_all;
           op.M = (int64_t)rA.size();
           op.N = (int64_t)cA.size();
           // same layout on both sides: a flat (vectorised) axpy
@@ -1433,6 +1589,13 @@ struct ccq_hip_ctx {
   bool group = true;
   // CCSDt triples residuals on the P space only (CCQ_HIP_PSPACE=0: full blocks)
   bool pspace = true;
+  // point-group blocking: 0-based irrep of every correlated orbital (abelian
+  // groups up to D2h, products by XOR); every tensor of the programs is
+  // totally symmetric, so a product C = A B splits into the blocks of rows,
+  // summed indices and columns of one irrep each (CCQ_HIP_SYM=0: dense)
+  std::vector<int> orb_irrep;
+  bool use_sym = false;
+  int part_count = 0;
   bool dump_schedule = false;   // CCQ_HIP_SCHEDULE=1: levels and launches to stderr
   static constexpr int64_t kGroupTiles = 512;   // below this a GEMM joins groups
   // products with K >= group_min_k join groups; small register-direct
@@ -1440,6 +1603,7 @@ struct ccq_hip_ctx {
   // (CCQ_HIP_GROUP_MINK, CCQ_HIP_GROUP_REG=0; both cut launches of the
   // latency-bound sweeps: CCSDT 40 orbitals 7.70 -> 7.37 ms)
   int64_t group_min_k = 16;
+  static constexpr int64_t kPartGroupMaxK = 1024;   // longest K of a part in a grouped launch
   bool group_reg = true;
 
   std::pair<uintptr_t, uintptr_t> span(int t) const {
@@ -1453,6 +1617,7 @@ struct ccq_hip_ctx {
   // the op's plan as a member of a grouped launch: its tile, full K in
   // 32-deep slabs, no split
   static ccqk::GemmArgs group_plan(const Op& op) {
+    if (op.has_gplan) return op.gplan;
     ccqk::GemmArgs g = op.plan;
     if (g.kind != 0) {
       // a register-direct product as a 32 x 32 tile member
@@ -1507,20 +1672,54 @@ struct ccq_hip_ctx {
       std::vector<int> level(n, 0);
       int top = 0;
       for (int q = 0; q < n; ++q) {
-        for (int r = 0; r < q; ++r)
+        const int pq = sec.ops[idx[q]].part_of;
+        for (int r = 0; r < q; ++r) {
+          // parts of one product write disjoint elements and read the same operands
+          if (pq >= 0 && sec.ops[idx[r]].part_of == pq) continue;
           if (level[r] >= level[q] &&
               (meet(wr[r], rd[q]) || meet(wr[r], wr[q]) || meet(rd[r], wr[q])))
             level[q] = level[r] + 1;
+        }
         top = std::max(top, level[q]);
       }
       for (int lv = 0; lv <= top; ++lv) {
         std::vector<int> here;
         for (int q = 0; q < n; ++q)
           if (level[q] == lv) here.push_back(idx[q]);
+        std::map<int, std::vector<int>> group_at;   // first member -> members
+        std::set<int> grouped;
+        // the parts of one product (irrep blocks, P-space rectangles) share a
+        // tile plan and run as grouped launches of up to kMaxGroup parts
+        {
+          std::map<int, std::vector<int>> sib;
+          for (int i : here) {
+            const Op& op = sec.ops[i];
+            if (op.gemm && op.chain < 0 && op.part_of >= 0 && op.has_gplan) sib[op.part_of].push_back(i);
+          }
+          for (int i : here) {
+            const Op& op = sec.ops[i];
+            if (!op.gemm && op.part_of >= 0) sib[-2 - op.part_of].push_back(i);   // add parts
+          }
+          for (auto& kv : sib) {
+            auto& v = kv.second;
+            if (v.size() < 2) continue;
+            const size_t cap = kv.first >= 0 ? (size_t)ccqk::kMaxGroup : (size_t)ccqk::kMaxPermGroup;
+            for (size_t lo = 0; lo < v.size(); lo += cap) {
+              const size_t hi = std::min(v.size(), lo + cap);
+              std::vector<int> members(v.begin() + lo, v.begin() + hi);
+              if (members.size() < 2) {
+                continue;
+              }
+              group_at[*std::min_element(members.begin(), members.end())] = members;
+              grouped.insert(members.begin(), members.end());
+            }
+          }
+        }
         // candidates by kernel instantiation
         std::map<int64_t, std::vector<int>> cand;
         for (int i : here) {
           const Op& op = sec.ops[i];
+          if (grouped.count(i)) continue;
           if (!op.gemm || op.chain >= 0 || op.K < group_min_k) continue;
           if (op.plan.kind != 0 && !(group_reg && op.K <= 2048 && op.M * op.N <= 65536)) continue;
           const ccqk::GemmArgs g = group_plan(op);
@@ -1544,8 +1743,6 @@ struct ccq_hip_ctx {
             a = b;
           }
         }
-        std::map<int, std::vector<int>> group_at;   // first member -> members
-        std::set<int> grouped;
         for (const auto& v : runs) {
           int64_t tiles = 0;
           for (int i : v) tiles += ccqk::gemm_tiles(group_plan(sec.ops[i]));
@@ -1588,7 +1785,8 @@ struct ccq_hip_ctx {
           if (it != group_at.end()) {
             sec.launches.push_back(it->second);
             sec.launch_cls.push_back(sec.ops[i].gemm ? group_cls(sec, it->second)
-                                                     : "add:flat*" + std::to_string(it->second.size()));
+                                     : (sec.ops[i].part_of >= 0 ? "add:perm*" : "add:flat*") +
+                                           std::to_string(it->second.size()));
             sec.launch_level.push_back(lv);
           } else if (!grouped.count(i)) {
             sec.launches.push_back({i});
@@ -1689,6 +1887,29 @@ struct ccq_hip_ctx {
       gs[k].group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, std::lround(std::sqrt(per_xcd))));
     }
     HIPCHK(ccqk::launch_gemm_group(stream, gs, n));
+  }
+
+  void run_perm_group(const Section& sec, const std::vector<int>& members) {
+    ccqk::PermGroup pg{};
+    if (members.size() > (size_t)ccqk::kMaxPermGroup) throw Fail("grouped permuted add too large");
+    pg.n = (int)members.size();
+    int64_t total = 0;
+    for (int k = 0; k < pg.n; ++k) {
+      const Op& op = sec.ops[members[k]];
+      pg.A[k] = tensors[op.a].ptr;
+      pg.rowA[k] = tables32 + op.t_rowA;
+      pg.colA[k] = tables32 + op.t_kA;
+      pg.C[k] = tensors[op.out].ptr;
+      pg.rowC[k] = tables32 + op.t_rowC;
+      pg.colC[k] = tables32 + op.t_colC;
+      pg.N[k] = op.N;
+      pg.alpha[k] = op.coef;
+      pg.accumulate[k] = op.assign ? 0 : 1;
+      pg.start[k] = total;
+      total += op.M * op.N;
+    }
+    pg.start[pg.n] = total;
+    HIPCHK(ccqk::launch_perm_group(stream, pg));
   }
 
   void run_add_group(const Section& sec, const std::vector<int>& members) {
@@ -1970,6 +2191,8 @@ struct ccq_hip_ctx {
             for (int i : L) fl += sec.ops[i].flops, by += sec.ops[i].bytes;
             if (sec.ops[L[0]].gemm)
               timed(sec.launch_cls[k], fl, by, [&] { run_group(sec, L); });
+            else if (sec.ops[L[0]].part_of >= 0)
+              timed(sec.launch_cls[k], fl, by, [&] { run_perm_group(sec, L); });
             else
               timed(sec.launch_cls[k], fl, by, [&] { run_add_group(sec, L); });
           };
@@ -2604,6 +2827,23 @@ int ccq_hip_set_ints(ccq_hip_ctx* ctx, const double* f_a, const double* f_b, con
     }
     ctx->gather_views();
     ctx->ints_ready = true;
+  });
+}
+
+int ccq_hip_set_irreps(ccq_hip_ctx* ctx, const int64_t* irreps, int64_t n) {
+  return guarded(ctx, [&] {
+    if (ctx->ints_ready) throw Fail("ccq_hip_set_irreps must come before ccq_hip_set_ints");
+    if (!irreps || n != ctx->orbs) throw Fail("ccq_hip_set_irreps: expected one irrep per orbital");
+    ctx->orb_irrep.assign((size_t)ctx->nc, 0);
+    bool any = false;
+    for (int64_t p = 0; p < ctx->nc; ++p) {
+      const int64_t v = irreps[ctx->froz + p];
+      if (v < 1 || v > 8) throw Fail("ccq_hip_set_irreps: irreps are 1..8");
+      ctx->orb_irrep[(size_t)p] = (int)(v - 1);
+      any = any || v != 1;
+    }
+    const char* g = std::getenv("CCQ_HIP_SYM");
+    ctx->use_sym = any && !(g && std::string(g) == "0");
   });
 }
 

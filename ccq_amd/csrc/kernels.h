@@ -179,6 +179,23 @@ struct AddGroup {
   int n;
 };
 hipError_t launch_axpy_group(hipStream_t s, const AddGroup& ag);
+// up to kMaxPermGroup permuted adds C_i(perm) (+)= alpha_i A_i(perm) with
+// their own offset tables in one launch (the irrep blocks of one add)
+constexpr int kMaxPermGroup = 8;
+struct PermGroup {
+  const double* A[kMaxPermGroup];
+  const int* rowA[kMaxPermGroup];
+  const int* colA[kMaxPermGroup];
+  double* C[kMaxPermGroup];
+  const int* rowC[kMaxPermGroup];
+  const int* colC[kMaxPermGroup];
+  int64_t N[kMaxPermGroup];
+  double alpha[kMaxPermGroup];
+  int accumulate[kMaxPermGroup];
+  int64_t start[kMaxPermGroup + 1];   // first element (row-major M x N) of each member
+  int n;
+};
+hipError_t launch_perm_group(hipStream_t s, const PermGroup& pg);
 hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
                            const int* colA, double* C, const int* rowC, const int* colC,
                            double alpha, int accumulate);

@@ -843,6 +843,22 @@ __global__ __launch_bounds__(256) void axpy_group(AddGroup ag) {
   }
 }
 
+__global__ __launch_bounds__(256) void perm_group(PermGroup pg) {
+  const int64_t total = pg.start[pg.n];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int m = 0;
+#pragma unroll
+    for (int i = 1; i < kMaxPermGroup; ++i)
+      if (i < pg.n && e >= pg.start[i]) m = i;
+    const int64_t k = e - pg.start[m];
+    const int64_t r = k / pg.N[m], c = k - r * pg.N[m];
+    const double v = pg.alpha[m] * pg.A[m][(int64_t)pg.rowA[m][r] + pg.colA[m][c]];
+    double* p = pg.C[m] + (int64_t)pg.rowC[m][r] + pg.colC[m][c];
+    if (pg.accumulate[m]) *p += v; else *p = v;
+  }
+}
+
 __global__ void copy_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (int64_t)gridDim.x * blockDim.x)
@@ -1686,6 +1702,14 @@ hipError_t launch_gemm_group(hipStream_t s, const GemmArgs* gs, int n) {
     case 32032: launch_group<32, 32>(s, gg); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_perm_group(hipStream_t s, const PermGroup& pg) {
+  if (pg.n < 1 || pg.n > kMaxPermGroup) return hipErrorInvalidValue;
+  const int64_t total = pg.start[pg.n];
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(perm_group, dim3(grid_for(total, 256, 8192)), dim3(256), 0, s, pg);
   return hipGetLastError();
 }
 

@@ -102,6 +102,7 @@ EXPORTS = {
     "ccq_hip_mfma_ceiling": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double)]),
+    "ccq_hip_set_irreps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]),
     "ccq_hip_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "ccq_hip_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "ccq_hip_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -267,6 +268,12 @@ class Engine:
 
     def profile(self, enable: bool):
         self._check(self.lib.ccq_hip_profile(self.h, int(enable)), "profile")
+
+    def set_irreps(self, irreps):
+        """Orbital irreps, 1-based as ccq's sym_file (src/symmetry.f90:84-114),
+        one per orbital; before set_ints.  Products then run in irrep blocks."""
+        ir = np.ascontiguousarray(irreps, dtype=np.int64)
+        self._check(self.lib.ccq_hip_set_irreps(self.h, _ptr(ir), ir.size), "set_irreps")
 
     def profile_report(self, sections: bool = False) -> dict:
         """Per launch class {calls, ms, flops, bytes} of the profiled launches;

@@ -71,6 +71,13 @@ int ccq_hip_set_ints(ccq_hip_ctx* ctx, const double* f_a, const double* f_b,
 /* Number of doubles in ccq's t_vec: cc%t_size = pos(10)-1 (get_t_sizes,
  * src/utils/cc_utils.f90:479-559), or pos(16)-1 with the ten P-space T3
  * blocks of CCSDt (get_t_sizes_act, :560-631). */
+/* Orbital irreps (ccq's orb_syms from read_sym, src/symmetry.f90:84-114;
+ * 1-based, products of the abelian groups C1 .. D2h by XOR of irrep - 1),
+ * one per orbital 1..orbs, before ccq_hip_set_ints.  ccq reads them but
+ * computes dense blocks; here every product is split into its irrep blocks
+ * (run as one grouped launch) and symmetry-forbidden elements are never
+ * computed; the results equal the dense ones.  No call: dense (C1). */
+int ccq_hip_set_irreps(ccq_hip_ctx* ctx, const int64_t* irreps, int64_t n);
 int64_t ccq_hip_t_size(const ccq_hip_ctx* ctx);
 
 /* Copy cc%t_vec to / from the device (T1..T3; T4 stays resident). */

@@ -192,6 +192,7 @@ def test_ccsdt_act_n80_d2h_zeros_and_cct3_tail():
     ao, au = 3, 8
     act = A.ActiveSpace(0, o, o, n, ao, au)
     eng = E.Engine(0, o, o, n, "CC(t;3)", act_occ=ao, act_unocc=au)
+    eng.set_irreps(np.asarray(s.irreps) + 1)    # D2h blocks (ccq's sym_file, 1-based)
     eng.set_ints(*ints)
     assert eng.t_size == 2 * o * v + 2 * (o * o * v * v) + (o * o * v * v) + \
         A.packed_t3_size(act, o, o, v, v)
