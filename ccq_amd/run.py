@@ -32,6 +32,8 @@ def main(argv=None):
     ap.add_argument("--cct3", action="store_true", help="CC(t;3) tail after CCSDt")
     ap.add_argument("--mixed", action="store_true", help="FP32 contractions + FP64 refinement")
     ap.add_argument("--guess", choices=("zero", "mp2", "learned"), default="zero")
+    ap.add_argument("--dense", action="store_true",
+                    help="ignore the workload's orbital irreps (no point-group blocks)")
     args = ap.parse_args(argv)
     spec = synth.WORKLOADS[args.workload]
     method = spec["method"]
@@ -48,6 +50,8 @@ def main(argv=None):
     t0 = time.perf_counter()
     eng = E.Engine(0, o, o, n, method, diis_space=args.diis, act_occ=args.act_occ,
                    act_unocc=args.act_unocc)
+    if s.irreps is not None and not args.dense:
+        eng.set_irreps(s.irreps + 1)     # ccq's sym_file numbering (1-based)
     eng.set_ints(ints.f_a, ints.f_b, ints.v_aa, ints.v_ab, ints.v_bb)
     print(f"engine created and integrals uploaded in {time.perf_counter() - t0:.1f}s", flush=True)
     if args.guess != "zero":

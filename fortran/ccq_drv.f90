@@ -17,7 +17,7 @@ program ccq_drv
     implicit none
 
     integer, parameter :: dp = c_double
-    character(len=512) :: inp, line, key, val, onebody, twobody, calc_type
+    character(len=512) :: inp, line, key, val, onebody, twobody, calc_type, sym_file
     integer(c_int64_t) :: froz_spin, nel, nvirt, mult, froz, occ_a, occ_b, orbs
     integer(c_int64_t) :: diis_space, max_iter, level, rhf_flag, act_occ, act_unocc
     real(dp) :: tol, shift
@@ -25,7 +25,9 @@ program ccq_drv
     real(dp), allocatable :: v_aa(:,:,:,:), v_ab(:,:,:,:)
     real(dp) :: en_repul, en_ref, e_cor, e_left, e23(4), t0
     integer(c_int64_t) :: i, j, k, a, b, ios, unit_in, itol, eq, nacc
-    logical :: rhf, converged, cct3
+    logical :: rhf, converged, cct3, sym_exists
+    integer(c_int64_t), allocatable :: orb_syms(:)
+    integer(c_int64_t) :: point_group, unit_sym
     real :: accv(5)
     type(ccq_hip_dims) :: dims
     type(ccq_hip_opts) :: opts
@@ -41,7 +43,7 @@ program ccq_drv
     froz_spin = 0; nel = -1; nvirt = -1; mult = 1; act_occ = -1; act_unocc = -1
     cct3 = .false.
     tol = 1.0e-4_dp; diis_space = 5; max_iter = 60; shift = 0.0_dp; rhf = .false.
-    onebody = 'onebody.inp'; twobody = 'twobody.inp'; calc_type = 'CCSD'
+    onebody = 'onebody.inp'; twobody = 'twobody.inp'; calc_type = 'CCSD'; sym_file = ''
 
     open(newunit=unit_in, file=trim(inp), status='old', action='read')
     do
@@ -69,6 +71,7 @@ program ccq_drv
             else
                 tol = 10.0_dp ** itol
             end if
+        case ('symfile', 'sym_file'); sym_file = val
         case ('onebody'); onebody = val
         case ('twobody'); twobody = val
         case ('calc_type'); calc_type = val
@@ -203,6 +206,26 @@ program ccq_drv
     if (ccq_hip_create(dims, opts, ctx) /= 0) then
         write(0, '(a)') 'ERROR in ccq_hip_create'
         call exit(1)
+    end if
+    ! orbital irreps (read_sym, src/symmetry.f90:15-119: the point group, then
+    ! one irrep per orbital; a missing file is C1): the engine runs every
+    ! product in its irrep blocks
+    if (len_trim(sym_file) > 0) then
+        inquire(file=trim(sym_file), exist=sym_exists)
+        if (sym_exists) then
+            allocate(orb_syms(orbs))
+            open(newunit=unit_sym, file=trim(sym_file), status='old', action='read')
+            read(unit_sym, *) point_group
+            if (point_group < 1 .or. point_group > 4) then
+                write(0, '(a)') 'ERROR in read_sym: Point-group symmetry not supported'
+                call exit(1)
+            end if
+            do i = 1, orbs
+                read(unit_sym, *) orb_syms(i)
+            end do
+            close(unit_sym)
+            call hip_check(ctx, ccq_hip_set_irreps(ctx, orb_syms, orbs), 'set_irreps')
+        end if
     end if
     call hip_check(ctx, ccq_hip_set_ints(ctx, f_a, f_b, v_aa, v_ab, v_aa), 'set_ints')
 

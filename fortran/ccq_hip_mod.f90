@@ -38,6 +38,13 @@ module ccq_hip_mod
             real(c_double), intent(in) :: f_a(*), f_b(*), v_aa(*), v_ab(*), v_bb(*)
         end function ccq_hip_set_ints
 
+        integer(c_int) function ccq_hip_set_irreps(ctx, irreps, n) bind(c, name='ccq_hip_set_irreps')
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int64_t), intent(in) :: irreps(*)
+            integer(c_int64_t), value :: n
+        end function ccq_hip_set_irreps
+
         integer(c_int64_t) function ccq_hip_t_size(ctx) bind(c, name='ccq_hip_t_size')
             import :: c_int64_t, c_ptr
             type(c_ptr), value :: ctx

@@ -71,3 +71,28 @@ def test_driver_cct3_matches_oracle(tmp_path):
     for name, k in (("CC(t;3),A", 0), ("CC(t;3),B", 1), ("CC(t;3),C", 2), ("CC(t;3)", 3)):
         assert abs(rows[name] - (ecor + e23[k])) < 1e-9, name
     assert "Left Coupled cluster" in out
+
+
+@pytest.mark.skipif(not os.path.exists(DRV), reason="fortran/ccq_drv not built (make -C fortran)")
+def test_driver_sym_file_blocks_equal_dense(tmp_path):
+    """A D2h system with ccq's sym_file keyword (point group 4, one 1-based
+    irrep per orbital, src/symmetry.f90:84-114): the driver hands the irreps
+    to the engine, which runs every product in irrep blocks; the printed
+    CCSDT energy equals the run without the file (dense products)."""
+    import numpy as np
+    from ccq_amd import synth
+    print("case fortran driver sym_file ccsdt n10", flush=True)
+    s = synth.make_system(10, 3, 20, 17, d2h_irreps=True)
+    totals = []
+    for sym in (False, True):
+        d = tmp_path / ("sym" if sym else "dense")
+        opts = dict(rhf=".true.", tol="1.0e-10", max_iterations=80)
+        if sym:
+            d.mkdir()
+            (d / "sym.inp").write_text("4\n" + "\n".join(str(int(g) + 1) for g in s.irreps) + "\n")
+            opts["sym_file"] = str(d / "sym.inp")
+        inp = synth.write_inputs(str(d), s, "CCSDT", **opts)
+        ref, tot = run_ccq("CCSDT", inp)
+        totals.append(tot)
+    assert np.isfinite(totals[0])
+    assert abs(totals[1] - totals[0]) < 1e-9
