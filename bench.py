@@ -52,6 +52,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# tile shapes and split-K from the measuring autotuner (set CCQ_HIP_TUNE=0 for the model's plans)
+os.environ.setdefault("CCQ_HIP_TUNE", "1")
 
 from ccq_amd import engine as E  # noqa: E402
 from ccq_amd import guess  # noqa: E402

@@ -15,6 +15,7 @@
 #include <map>
 #include <set>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -1070,6 +1071,133 @@ struct ccq_hip_ctx {
     return t + "s" + std::to_string(g.nsplit);
   }
 
+  // Measuring autotuner.  For every product launched on its own, the
+  // planner's kTuneCandidates best plans by modelled cycles are each run
+  // once to warm up and kTuneReps times timed with HIP events on the
+  // context's stream (before the first sweep: products only write
+  // intermediates and residual accumulators, which every sweep recomputes),
+  // and the fastest is kept.  Results are shared by every product with the
+  // same shape and operand layouts for the life of the process, so every
+  // context of a process makes the same choices.
+  static constexpr int kTuneCandidates = 6;
+  static constexpr int kTuneReps = 3;
+
+  struct TuneCache {
+    std::mutex mu;
+    std::map<std::string, ccqk::GemmArgs> plans;
+  };
+  static TuneCache& tune_cache() {
+    static TuneCache c;
+    return c;
+  }
+
+  static std::string tune_key(const Op& op) {
+    const ccqk::GemmArgs& g = op.plan;
+    std::ostringstream k;
+    k << g.M << 'x' << g.N << 'x' << g.K << ':' << g.a_kfast << g.b_kfast << g.c_mfast << g.c_nfast << ':'
+      << g.kA_aff << ':' << g.kB_aff << ':' << (op.assign ? 0 : 1) << (op.t_rowC2 >= 0) << (op.t_colC2 >= 0);
+    return k.str();
+  }
+
+  static void copy_plan(ccqk::GemmArgs& dst, const ccqk::GemmArgs& src) {
+    dst.kind = src.kind;
+    dst.trans = src.trans;
+    dst.bm = src.bm;
+    dst.bn = src.bn;
+    dst.bk = src.bk;
+    dst.rm = src.rm;
+    dst.rn = src.rn;
+    dst.wk = src.wk;
+    dst.nsplit = src.nsplit;
+    dst.kchunk = src.kchunk;
+    dst.group = src.group;
+    dst.ksplit = src.ksplit;
+  }
+
+  void autotune(const std::vector<int>& host32) {
+    int* dt = nullptr;
+    HIPCHK(hipMalloc(&dt, sizeof(int) * std::max<size_t>(host32.size(), 1)));
+    HIPCHK(hipMemcpy(dt, host32.data(), sizeof(int) * host32.size(), hipMemcpyHostToDevice));
+    DevBuf tws;
+    tws.alloc(kWsCap);
+    int* saved_tab = tables32;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    tables32 = dt;
+    cur_ws = tws.p;
+    auto restore = [&] {
+      tables32 = saved_tab;
+      cur_ws = nullptr;
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      (void)hipFree(dt);
+    };
+    try {
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      TuneCache& cache = tune_cache();
+      for (auto& sec : sections) {
+        if (sec.kind != Section::Compute) continue;
+        for (auto& op : sec.ops) {
+          if (!op.gemm || op.has_gplan || op.K == 0) continue;
+          const std::string key = tune_key(op);
+          {
+            std::lock_guard<std::mutex> lk(cache.mu);
+            auto hit = cache.plans.find(key);
+            if (hit != cache.plans.end()) {
+              copy_plan(op.plan, hit->second);
+              continue;
+            }
+          }
+          std::vector<std::pair<double, ccqk::GemmArgs>> ranked;
+          ccqk::GemmArgs model = op.plan;
+          ccqk::gemm_plan_ranked(model, kWsCap, &ranked);
+          std::vector<ccqk::GemmArgs> cands;
+          for (auto& rc : ranked) {
+            const ccqk::GemmArgs& c = rc.second;
+            bool dup = false;
+            for (auto& d : cands)
+              dup = dup || (d.kind == c.kind && d.trans == c.trans && d.bm == c.bm && d.bn == c.bn &&
+                            d.bk == c.bk && d.rm == c.rm && d.rn == c.rn && d.wk == c.wk &&
+                            d.nsplit == c.nsplit && d.ksplit == c.ksplit);
+            if (!dup) cands.push_back(c);
+            if ((int)cands.size() == kTuneCandidates) break;
+          }
+          double best_t = 1e300;
+          ccqk::GemmArgs best = op.plan;
+          for (auto& c : cands) {
+            Op trial = op;
+            copy_plan(trial.plan, c);
+            ccqk::GemmArgs a = gemm_args(trial);
+            HIPCHK(ccqk::launch_gemm(stream, a));
+            HIPCHK(hipEventRecord(e0, stream));
+            for (int r = 0; r < kTuneReps; ++r) HIPCHK(ccqk::launch_gemm(stream, a));
+            HIPCHK(hipEventRecord(e1, stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_t) {
+              best_t = ms;
+              best = c;
+            }
+          }
+          copy_plan(op.plan, best);
+          std::lock_guard<std::mutex> lk(cache.mu);
+          cache.plans.emplace(key, best);
+        }
+      }
+    } catch (...) {
+      restore();
+      throw;
+    }
+    restore();
+    // class names follow the chosen plans
+    for (auto& sec : sections)
+      for (auto& op : sec.ops)
+        if (op.gemm)
+          op.cls = "gemm:" + std::to_string(op.M) + "x" + std::to_string(op.N) + "x" + std::to_string(op.K) +
+                   "/" + plan_tag(op.plan) + (op.pack_k || op.t_rowC2 >= 0 || op.t_colC2 >= 0 ? "p" : "");
+  }
+
   void plan() {
     std::vector<int> host32;
     auto push32 = [&](const std::vector<int64_t>& v) {
@@ -1357,6 +1485,13 @@ struct ccq_hip_ctx {
       }
       sec.ops = std::move(planned);
     }
+    if (tune) {
+      autotune(host32);
+      ws_need = 0;
+      for (auto& sec : sections)
+        for (auto& op : sec.ops)
+          if (op.gemm) ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(op.plan));
+    }
     fuse_adds(push32);
     // groups first; products left alone are chained where they can be
     group_gemms(false);
@@ -1589,6 +1724,8 @@ struct ccq_hip_ctx {
   bool group = true;
   // CCSDt triples residuals on the P space only (CCQ_HIP_PSPACE=0: full blocks)
   bool pspace = true;
+  // measuring autotuner over the planner's best candidate plans (CCQ_HIP_TUNE=1)
+  bool tune = false;
   // point-group blocking: 0-based irrep of every correlated orbital (abelian
   // groups up to D2h, products by XOR); every tensor of the programs is
   // totally symmetric, so a product C = A B splits into the blocks of rows,
@@ -2725,6 +2862,7 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (const char* g = std::getenv("CCQ_HIP_VIEWPERM")) ctx->view_perm = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_SCHEDULE")) ctx->dump_schedule = std::string(g) == "1";
     if (const char* g = std::getenv("CCQ_HIP_PSPACE")) ctx->pspace = std::string(g) != "0";
+    if (const char* g = std::getenv("CCQ_HIP_TUNE")) ctx->tune = std::string(g) == "1";
     if (const char* g = std::getenv("CCQ_HIP_GROUP_MINK")) ctx->group_min_k = std::max(1, std::atoi(g));
     if (const char* g = std::getenv("CCQ_HIP_GROUP_REG")) ctx->group_reg = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_STREAMS"))

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+#include <vector>
+
 namespace ccqk {
 
 constexpr int kMaxRank = 8;
@@ -129,6 +132,9 @@ struct Coeffs {
 // Choose kernel, tile shape and split-K for a GEMM; ws_cap bounds the
 // split-K workspace (doubles).
 void gemm_plan(GemmArgs& g, int64_t ws_cap);
+// the same, also returning every candidate plan with its modelled cycles,
+// best first (the input for the measuring autotuner)
+void gemm_plan_ranked(GemmArgs& g, int64_t ws_cap, std::vector<std::pair<double, GemmArgs>>* ranked);
 // Output tiles of the planned launch, and split-K workspace (doubles) it
 // needs (0 when nsplit == 1).
 int64_t gemm_tiles(const GemmArgs& g);

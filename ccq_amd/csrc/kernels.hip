@@ -20,6 +20,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <vector>
 #include <cmath>
 
 #include "kernels.h"
@@ -1352,7 +1353,12 @@ static double splitk_cycles(int64_t M, int64_t N, int64_t split) {
   return 2.0 * split * M * N * 8.0 / hbm + 8000.0;
 }
 
-void gemm_plan(GemmArgs& g, int64_t ws_cap) {
+void gemm_plan(GemmArgs& g, int64_t ws_cap) { gemm_plan_ranked(g, ws_cap, nullptr); }
+
+// The planner's cycle model over every kernel / tile / split-K choice; the
+// best one goes into g, and every candidate with its modelled cycles into
+// ranked when given (the autotuner measures the best few).
+void gemm_plan_ranked(GemmArgs& g, int64_t ws_cap, std::vector<std::pair<double, GemmArgs>>* ranked) {
   // Cost model in cycles (2.4 GHz).  A 16x16x4 FP64 MFMA occupies a SIMD for
   // 64 cycles (78.6 TF/s over 256 CUs x 4 SIMDs).
   const double cus = 256.0;
@@ -1364,7 +1370,22 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
   const double hbm_reg = 2.5e12 / 2.4e9;
   double best = 1e300;
   auto take = [&](double cyc, int kind, int trans, int bm, int bn, int bk, const RegShape* r,
-                  int64_t split, int64_t kchunk) {
+                  int64_t split, int64_t kchunk, int ksp) {
+    if (ranked) {
+      GemmArgs c = g;
+      c.kind = kind;
+      c.trans = trans;
+      c.bm = bm;
+      c.bn = bn;
+      c.bk = bk;
+      c.rm = r ? r->rm : 1;
+      c.rn = r ? r->rn : 1;
+      c.wk = r ? r->wk : 1;
+      c.nsplit = (int)split;
+      c.kchunk = kchunk;
+      c.ksplit = ksp;
+      ranked->emplace_back(cyc, c);
+    }
     if (cyc >= best) return;
     best = cyc;
     g.kind = kind;
@@ -1429,7 +1450,7 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
         }
         cyc += splitk_cycles(g.M, g.N, split);
         const double before = best;
-        take(cyc, 0, ttile, sh.bm, sh.bn, bk, nullptr, split, steps * ksp * bk);
+        take(cyc, 0, ttile, sh.bm, sh.bn, bk, nullptr, split, steps * ksp * bk, ksp);
         if (best < before) g.ksplit = ksp;
       }
     }
@@ -1463,17 +1484,27 @@ void gemm_plan(GemmArgs& g, int64_t ws_cap) {
         double cyc = std::max(lat, bytes / hbm_reg) + 0.05 * lat;
         cyc += splitk_cycles(g.M, g.N, split);
         const double before = best;
-        take(cyc, 1, trans, 64, 64, 16, &r, split, cdiv(k16, split) * 16);
+        take(cyc, 1, trans, 64, 64, 16, &r, split, cdiv(k16, split) * 16, 1);
         if (best < before) g.ksplit = 1;
       }
     }
   }
   // tile-row group of the XCD-aware order: about square per-XCD blocks
-  int64_t tbm, tbn;
-  tile_dims(g, tbm, tbn);
-  const int64_t tm = cdiv(g.trans ? g.N : g.M, tbm);
-  const double per_xcd = (double)gemm_tiles(g) * g.nsplit / 8.0;
-  g.group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, (int64_t)std::lround(std::sqrt(per_xcd))));
+  auto set_group = [](GemmArgs& x) {
+    int64_t tbm, tbn;
+    tile_dims(x, tbm, tbn);
+    const int64_t tm = cdiv(x.trans ? x.N : x.M, tbm);
+    const double per_xcd = (double)gemm_tiles(x) * x.nsplit / 8.0;
+    x.group = (int)std::max<int64_t>(1, std::min<int64_t>(tm, (int64_t)std::lround(std::sqrt(per_xcd))));
+  };
+  set_group(g);
+  if (ranked) {
+    for (auto& c : *ranked) set_group(c.second);
+    std::stable_sort(ranked->begin(), ranked->end(),
+                     [](const std::pair<double, GemmArgs>& a, const std::pair<double, GemmArgs>& b) {
+                       return a.first < b.first;
+                     });
+  }
 }
 
 template <typename T, int BM, int BN, int BK, int KS>

@@ -340,3 +340,29 @@ def test_pspace_products_match_full_blocks(n, o, ob, ao, au, monkeypatch):
     assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
     if ao < o or au < c.va:
         assert f1 < f0
+
+
+@pytest.mark.parametrize("method,n,o,ob", [("CCSD", 24, 5, 4), ("CCSDT", 12, 3, 2)])
+def test_autotuned_plans_match_model_plans(method, n, o, ob, monkeypatch):
+    """The measuring autotuner (CCQ_HIP_TUNE=1) only changes tile shapes and
+    split-K of the products: three sweeps agree with the model's plans to
+    1e-11, and a second context of the process reuses the first one's
+    choices (bit-identical sweeps)."""
+    _say(f"case autotuned plans {method} n={n} o={o} ob={ob}")
+    c = Case(n, o, 2 * n, 61, occ_b=ob)
+    vec = _state(c, method, 29)
+    out = []
+    for tune in ("0", "1", "1"):
+        eng = _engine_env(c, method, monkeypatch, {"CCQ_HIP_TUNE": tune, "CCQ_HIP_GRAPH": "0"}, shift=0.02)
+        eng.set_t(vec)
+        es = []
+        for _ in range(3):
+            eng.update()
+            es.append(eng.energy())
+        out.append((np.array(es), eng.get_t()))
+        eng.close()
+    (e0, t0), (e1, t1), (e2, t2) = out
+    np.testing.assert_allclose(e1, e0, rtol=1e-11, atol=0)
+    assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
+    np.testing.assert_array_equal(e2, e1)
+    np.testing.assert_array_equal(t2, t1)
