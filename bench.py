@@ -188,18 +188,21 @@ def cpu_baseline(s, ints, method, t_vec, n_iter):
 
 def measured_traffic(kernel_class: str):
     """HBM bytes per launch of a kernel class from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summaries under profiles/ (None if not measured)."""
+    WRITE_SIZE summaries under profiles/ (None if not measured).  A summary
+    names the class with its plan (gemm:MxNxK/plan) or by its shape alone
+    (gemm:MxNxK, for plans chosen by the autotuner); the exact name wins."""
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None
+    shape = kernel_class.split("/")[0]
+    found = {}
     for name in sorted(os.listdir(pdir)):
         with open(os.path.join(pdir, name)) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and \
-                        parts[1] == kernel_class:
-                    return float(parts[2])
-    return None
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch":
+                    found[parts[1]] = float(parts[2])
+    return found.get(kernel_class, found.get(shape))
 
 
 def copy_probe(nbytes: int):
