@@ -80,6 +80,7 @@ struct Tensor {
   int64_t dims[8] = {0};
   int64_t strides[8] = {0};
   double* ptr = nullptr;
+  float* p32 = nullptr;      // float shadow of the FP32-storage sweeps
   std::string base;          // View: base array; Amp/Resid: block name
   std::string spec[8];       // View: per-index "oa", "vb", ...
   // View: where the block lives inside the full-range base array; the view is
@@ -235,8 +236,18 @@ struct ccq_hip_ctx {
   std::vector<Section> sections;
   bool ints_ready = false;
   // arithmetic of the LDS-tiled contractions of the sweep: 0 FP64 MFMA,
-  // 1 FP32 MFMA (mixed precision; everything else stays FP64)
+  // 1 FP32 MFMA (mixed precision; everything else stays FP64), 2 FP32 MFMA
+  // with FP32 storage: every tensor of the sweep has a float shadow (made at
+  // the first such sweep, zeroed), the sweep casts t_vec to its shadow
+  // first, and its products, adds and updates read and write the shadows;
+  // the updates still write the FP64 amplitudes.  Other programs run mode 2
+  // as mode 1.
   int prec = 0;
+  int run_prec = 0;          // mode of the program being run
+  bool f32_ready = false;
+  bool views32_stale = true;   // views gathered again since their last cast
+  float* tvec32 = nullptr;
+  std::vector<float*> f32_bufs;
   // CC(t;3) tail (level 6): H-bar, left-CCSD and the triples correction run
   // as programs of a second IR on the same resident tensors
   bool tail = false;
@@ -256,6 +267,7 @@ struct ccq_hip_ctx {
 
   ~ccq_hip_ctx() {
     for (auto& kv : graphs) drop_graph(kv.second);
+    for (float* p : f32_bufs) (void)hipFree(p);
     if (guess_params) (void)hipFree(guess_params);
     if (d3tab) (void)hipFree(d3tab);
     if (tables32) (void)hipFree(tables32);
@@ -814,6 +826,74 @@ struct ccq_hip_ctx {
                                    gather_tab + g.t_dst_col));
     }
     HIPCHK(hipStreamSynchronize(stream));
+    views32_stale = true;
+  }
+
+  // Float shadows of the tensors the sweep reads or writes (intermediates
+  // and residuals zeroed, so elements no product writes stay zero) and
+  // the cast of the views; outside any graph capture.
+  void prep_f32() {
+    if (!f32_ready) {
+      auto alloc32 = [&](int64_t n) {
+        float* p = nullptr;
+        const size_t bytes = sizeof(float) * (size_t)std::max<int64_t>(n, 1);
+        HIPCHK(hipMalloc(&p, bytes));
+        f32_bufs.push_back(p);
+        HIPCHK(hipMemset(p, 0, bytes));
+        return p;
+      };
+      tvec32 = alloc32(dev_t);
+      std::set<int> used;
+      for (const auto& sec : sections) {
+        if (sec.prog != "sweep") continue;
+        if (sec.kind == Section::Update) used.insert(sec.upd.resid);
+        for (const auto& op : sec.ops) {
+          for (int t : {op.out, op.a, op.b, op.fuse_a})
+            if (t >= 0) used.insert(t);
+          // an op into the amplitudes (the closed-shell permuted copies) runs
+          // on both storages, so it may read only the amplitudes
+          if (tensors[op.out].kind == TKind::Amp &&
+              (op.gemm || tensors[op.a].kind != TKind::Amp || op.fuse_a >= 0))
+            throw Fail("sweep op into " + tensors[op.out].id + " has no FP32-storage form");
+        }
+      }
+      std::map<double*, float*> shadow;
+      for (int ti : used) {
+        Tensor& t = tensors[ti];
+        if (t.kind == TKind::Amp) {
+          if (t.base[0] == 'l') throw Fail("left amplitudes in the sweep");
+          t.p32 = tvec32 + blk_off.at(t.base);
+          continue;
+        }
+        auto& s = shadow[t.ptr];
+        if (!s) {
+          int64_t n = 1;
+          for (int i = 0; i < t.rank; ++i) n *= t.dims[i];
+          s = alloc32(n);
+        }
+        t.p32 = s;
+      }
+      f32_ready = true;
+      views32_stale = true;
+    }
+    if (views32_stale) {
+      std::set<float*> done;
+      for (const auto& t : tensors) {
+        if (t.kind != TKind::View || !t.p32 || !done.insert(t.p32).second) continue;
+        int64_t n = 1;
+        for (int i = 0; i < t.rank; ++i) n *= t.dims[i];
+        HIPCHK(ccqk::launch_cast_f32(stream, t.p32, t.ptr, n));
+      }
+      HIPCHK(hipStreamSynchronize(stream));
+      views32_stale = false;
+    }
+  }
+
+  // storage of tensor t in the program being run
+  double* sp(int t) const {
+    if (run_prec != 2) return tensors[t].ptr;
+    if (!tensors[t].p32) throw Fail("tensor " + tensors[t].id + " has no float shadow");
+    return reinterpret_cast<double*>(tensors[t].p32);
   }
 
   // Offsets of the index combinations of `subset` (row-major in subset
@@ -1115,6 +1195,7 @@ struct ccq_hip_ctx {
   }
 
   void autotune(const std::vector<int>& host32) {
+    run_prec = 0;   // candidates are timed in FP64 on the FP64 tensors
     int* dt = nullptr;
     HIPCHK(hipMalloc(&dt, sizeof(int) * std::max<size_t>(host32.size(), 1)));
     HIPCHK(hipMemcpy(dt, host32.data(), sizeof(int) * host32.size(), hipMemcpyHostToDevice));
@@ -1970,25 +2051,22 @@ struct ccq_hip_ctx {
   // execution
 
   ccqk::GemmArgs gemm_args(const Op& op) const {
-    const Tensor& C = tensors[op.out];
-    const Tensor& A = tensors[op.a];
-    const Tensor& B = tensors[op.b];
     ccqk::GemmArgs g = op.plan;
-    g.A = A.ptr;
+    g.A = sp(op.a);
     g.rowA = tables32 + op.t_rowA;
     g.kA = tables32 + op.t_kA;
-    g.B = B.ptr;
+    g.B = sp(op.b);
     g.kB = tables32 + op.t_kB;
     g.colB = tables32 + op.t_colB;
-    g.C = C.ptr;
+    g.C = sp(op.out);
     g.rowC = tables32 + op.t_rowC;
     g.colC = tables32 + op.t_colC;
     g.alpha = op.coef;
     g.accumulate = op.assign ? 0 : 1;
     g.ws = cur_ws;
-    g.prec = prec;
+    g.prec = run_prec;
     if (op.fuse_a >= 0) {
-      g.D = tensors[op.fuse_a].ptr;
+      g.D = sp(op.fuse_a);
       g.rowD = tables32 + op.t_rowD;
       g.colD = tables32 + op.t_colD;
       g.dcoef = op.dcoef;
@@ -2027,16 +2105,19 @@ struct ccq_hip_ctx {
   }
 
   void run_perm_group(const Section& sec, const std::vector<int>& members) {
+    if (amp_in_fp64(members.size(), [&](size_t k) { return sec.ops[members[k]].out; },
+                    [&] { run_perm_group(sec, members); }))
+      return;
     ccqk::PermGroup pg{};
     if (members.size() > (size_t)ccqk::kMaxPermGroup) throw Fail("grouped permuted add too large");
     pg.n = (int)members.size();
     int64_t total = 0;
     for (int k = 0; k < pg.n; ++k) {
       const Op& op = sec.ops[members[k]];
-      pg.A[k] = tensors[op.a].ptr;
+      pg.A[k] = sp(op.a);
       pg.rowA[k] = tables32 + op.t_rowA;
       pg.colA[k] = tables32 + op.t_kA;
-      pg.C[k] = tensors[op.out].ptr;
+      pg.C[k] = sp(op.out);
       pg.rowC[k] = tables32 + op.t_rowC;
       pg.colC[k] = tables32 + op.t_colC;
       pg.N[k] = op.N;
@@ -2046,43 +2127,78 @@ struct ccq_hip_ctx {
       total += op.M * op.N;
     }
     pg.start[pg.n] = total;
-    HIPCHK(ccqk::launch_perm_group(stream, pg));
+    HIPCHK(ccqk::launch_perm_group(stream, pg, run_prec == 2));
+  }
+
+  // float shadow of amplitude block b = its FP64 values
+  void cast_block(const std::string& b) {
+    HIPCHK(ccqk::launch_cast_f32(stream, tvec32 + blk_off.at(b), tvec.p + blk_off.at(b), blk_size.at(b)));
+  }
+
+  // In FP32-storage sweeps an add into the amplitudes (the closed-shell
+  // permuted copies, which read amplitudes only) runs on the FP64 storage
+  // and its output blocks are cast to their float shadows; returns whether
+  // it ran the launch.
+  template <typename OutOf, typename Run>
+  bool amp_in_fp64(size_t n, OutOf out_of, Run run) {
+    if (run_prec != 2) return false;
+    std::set<std::string> blocks;
+    for (size_t k = 0; k < n; ++k)
+      if (tensors[out_of(k)].kind == TKind::Amp) blocks.insert(tensors[out_of(k)].base);
+    if (blocks.empty()) return false;
+    for (size_t k = 0; k < n; ++k)
+      if (tensors[out_of(k)].kind != TKind::Amp) throw Fail("launch mixes adds into amplitudes and others");
+    run_prec = 0;
+    try {
+      run();
+    } catch (...) {
+      run_prec = 2;
+      throw;
+    }
+    run_prec = 2;
+    for (const auto& b : blocks) cast_block(b);
+    return true;
   }
 
   void run_add_group(const Section& sec, const std::vector<int>& members) {
+    if (amp_in_fp64(members.size(), [&](size_t k) { return sec.ops[members[k]].out; },
+                    [&] { run_add_group(sec, members); }))
+      return;
     ccqk::AddGroup ag{};
     if (members.size() > (size_t)ccqk::kMaxAddGroup) throw Fail("grouped add launch too large");
     ag.n = (int)members.size();
     int64_t total = 0;
     for (int k = 0; k < ag.n; ++k) {
       const Op& op = sec.ops[members[k]];
-      ag.A[k] = tensors[op.a].ptr;
-      ag.C[k] = tensors[op.out].ptr;
+      ag.A[k] = sp(op.a);
+      ag.C[k] = sp(op.out);
       ag.alpha[k] = op.coef;
       ag.accumulate[k] = op.assign ? 0 : 1;
       ag.start[k] = total;
       total += op.M * op.N;
     }
     ag.start[ag.n] = total;
-    HIPCHK(ccqk::launch_axpy_group(stream, ag));
+    HIPCHK(ccqk::launch_axpy_group(stream, ag, run_prec == 2));
   }
 
   void run_op(const Op& op) {
     if (op.fused) return;
-    const Tensor& C = tensors[op.out];
-    const Tensor& A = tensors[op.a];
     const int acc = op.assign ? 0 : 1;
     if (op.gemm) {
       HIPCHK(ccqk::launch_gemm(stream, gemm_args(op)));
     } else {
+      if (amp_in_fp64(1, [&](size_t) { return op.out; }, [&] { run_op(op); })) return;
+      const int io32 = run_prec == 2;
+      const double* A = sp(op.a);
+      double* C = sp(op.out);
+      // the float form has no alignment condition
       const bool aligned =
-          ((reinterpret_cast<uintptr_t>(A.ptr) | reinterpret_cast<uintptr_t>(C.ptr)) & 15) == 0;
+          io32 || ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(C)) & 15) == 0;
       if (op.flat && aligned)
-        HIPCHK(ccqk::launch_axpy_flat(stream, op.M * op.N, A.ptr, C.ptr, op.coef, acc));
+        HIPCHK(ccqk::launch_axpy_flat(stream, op.M * op.N, A, C, op.coef, acc, io32));
       else
-        HIPCHK(ccqk::launch_perm_add(stream, op.M, op.N, A.ptr, tables32 + op.t_rowA,
-                                     tables32 + op.t_kA, C.ptr, tables32 + op.t_rowC,
-                                     tables32 + op.t_colC, op.coef, acc));
+        HIPCHK(ccqk::launch_perm_add(stream, op.M, op.N, A, tables32 + op.t_rowA, tables32 + op.t_kA, C,
+                                     tables32 + op.t_rowC, tables32 + op.t_colC, op.coef, acc, io32));
     }
   }
 
@@ -2147,8 +2263,13 @@ struct ccq_hip_ctx {
       return;
     }
     double* t = tvec.p + blk_off.at(u.block);
-    HIPCHK(ccqk::launch_update(stream, P, t, R.ptr, itab + u.dev_perm_off, dtab + u.dev_sign_off,
+    // FP32 storage: the residual is read as float; the updated block is then
+    // cast to its shadow in one coalesced pass (cheaper than scattered float
+    // stores beside the FP64 ones)
+    P.u32 = run_prec == 2;
+    HIPCHK(ccqk::launch_update(stream, P, t, sp(u.resid), itab + u.dev_perm_off, dtab + u.dev_sign_off,
                                fdiag_a.p, fdiag_b.p));
+    if (run_prec == 2) cast_block(u.block);
   }
 
   const Tensor& tens(const std::string& id) const {
@@ -2161,6 +2282,7 @@ struct ccq_hip_ctx {
     if (blk_size.at(dst) != blk_size.at(src)) throw Fail("copy between blocks of different size");
     HIPCHK(ccqk::launch_copy(stream, tvec.p + blk_off.at(dst), tvec.p + blk_off.at(src),
                              blk_size.at(src)));
+    if (run_prec == 2) cast_block(dst);
   }
 
   // A sweep (and a left-CC step) is a fixed sequence of kernel launches on
@@ -2182,12 +2304,13 @@ struct ccq_hip_ctx {
   }
 
   void run_graphed(const std::string& prog, double key = 0.0) {
+    if (prog == "sweep" && prec == 2) prep_f32();
     if (!use_graph || profile) {
       run_program(prog);
       return;
     }
     // one graph per program and arithmetic (the launches differ)
-    Graph& G = graphs[prog + (prec ? ":fp32" : "")];
+    Graph& G = graphs[prog + ":p" + std::to_string(prec)];
     if (G.x && G.key != key) drop_graph(G);
     if (!G.x) {
       HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -2208,7 +2331,10 @@ struct ccq_hip_ctx {
 
   void run_sweep() { run_graphed("sweep"); }
 
-  void sweep(int64_t nsec = -1) { run_program("sweep", nsec); }
+  void sweep(int64_t nsec = -1) {
+    if (prec == 2) prep_f32();
+    run_program("sweep", nsec);
+  }
 
   std::string cur_sec;   // section being run (profile attribution)
 
@@ -2270,6 +2396,11 @@ struct ccq_hip_ctx {
 
   void run_program(const std::string& prog, int64_t nsec = -1) {
     if (!ints_ready) throw Fail("integrals not uploaded");
+    run_prec = prog == "sweep" ? prec : std::min(prec, 1);
+    if (run_prec == 2) {
+      if (!f32_ready || views32_stale) throw Fail("FP32 shadows not prepared");
+      HIPCHK(ccqk::launch_cast_f32(stream, tvec32, tvec.p, dev_t));
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (profile) {
       HIPCHK(hipEventCreate(&e0));
@@ -3248,7 +3379,8 @@ int ccq_hip_guess_data(ccq_hip_ctx* ctx, float* X, float* M, float* Y, int64_t n
 
 int ccq_hip_set_precision(ccq_hip_ctx* ctx, int64_t mode) {
   return guarded(ctx, [&] {
-    if (mode != 0 && mode != 1) throw Fail("precision mode must be 0 (FP64) or 1 (FP32 MFMA)");
+    if (mode < 0 || mode > 2)
+      throw Fail("precision mode must be 0 (FP64), 1 (FP32 MFMA) or 2 (FP32 MFMA, FP32 storage)");
     ctx->prec = (int)mode;
   });
 }

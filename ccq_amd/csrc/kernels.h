@@ -85,6 +85,8 @@ struct UpdateParams {
   const int* coc = nullptr;
   const int* cvc = nullptr;
   int64_t n_vc = 1;
+  // FP32-storage sweeps: U holds floats
+  int u32 = 0;
 };
 
 // CC(t;3) triples correction of one spin case (crcc23, src/mm_correct.f90:11-343).
@@ -184,7 +186,8 @@ struct AddGroup {
   int64_t start[kMaxAddGroup + 1];   // first element of each member
   int n;
 };
-hipError_t launch_axpy_group(hipStream_t s, const AddGroup& ag);
+// io32: the arrays hold floats (FP32-storage sweeps)
+hipError_t launch_axpy_group(hipStream_t s, const AddGroup& ag, int io32 = 0);
 // up to kMaxPermGroup permuted adds C_i(perm) (+)= alpha_i A_i(perm) with
 // their own offset tables in one launch (the irrep blocks of one add)
 constexpr int kMaxPermGroup = 8;
@@ -201,10 +204,10 @@ struct PermGroup {
   int64_t start[kMaxPermGroup + 1];   // first element (row-major M x N) of each member
   int n;
 };
-hipError_t launch_perm_group(hipStream_t s, const PermGroup& pg);
+hipError_t launch_perm_group(hipStream_t s, const PermGroup& pg, int io32 = 0);
 hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
                            const int* colA, double* C, const int* rowC, const int* colC,
-                           double alpha, int accumulate);
+                           double alpha, int accumulate, int io32 = 0);
 // Y[rowY[m] + colY[n]] = scale * X[rowX[m] + colX[n]] (int64 tables; view
 // set-up, packing of the active-space t_vec)
 hipError_t launch_gather64(hipStream_t s, int64_t M, int64_t N, const double* X,
@@ -212,7 +215,9 @@ hipError_t launch_gather64(hipStream_t s, int64_t M, int64_t N, const double* X,
                            const int64_t* rowY, const int64_t* colY, double scale = 1.0);
 // C (+)= alpha A for identically laid out dense arrays of n doubles (16-byte aligned)
 hipError_t launch_axpy_flat(hipStream_t s, int64_t n, const double* A, double* C, double alpha,
-                            int accumulate);
+                            int accumulate, int io32 = 0);
+// y = (float)x (the FP32 shadows of the FP32-storage sweeps)
+hipError_t launch_cast_f32(hipStream_t s, float* y, const double* x, int64_t n);
 hipError_t launch_fill(hipStream_t s, double* x, int64_t n, double v);
 hipError_t launch_copy(hipStream_t s, double* y, const double* x, int64_t n);
 // MFMA issue-rate probe; out holds blocks * 256 doubles; *flops = work of the launch

@@ -39,25 +39,40 @@ __device__ __forceinline__ int64_t koff(const int* __restrict__ tab, int64_t aff
   return aff >= 0 ? k * aff : (int64_t)tab[k];
 }
 
-__device__ __forceinline__ void put_c(double* p, double v, int accumulate) {
-  if (accumulate) *p += v; else *p = v;
+// Storage type IO of the tensors a kernel reads and writes: double, or float
+// in the FP32-storage sweeps (the argument pointers then hold float arrays;
+// arithmetic on loaded values is FP64 in the epilogues)
+template <typename IO>
+__device__ __forceinline__ double ldg(const double* p, int64_t i) {
+  return (double)reinterpret_cast<const IO*>(p)[i];
+}
+template <typename IO>
+__device__ __forceinline__ void stg(double* p, int64_t i, double v) {
+  reinterpret_cast<IO*>(p)[i] = (IO)v;
 }
 
+template <typename IO>
+__device__ __forceinline__ void put_c(double* base, int64_t off, double v, int accumulate) {
+  if (accumulate) stg<IO>(base, off, ldg<IO>(base, off) + v); else stg<IO>(base, off, v);
+}
+
+template <typename IO>
 __device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n, double s, double alpha) {
   const int64_t r = g.rowC[m], c = g.colC[n];
   double v = alpha * s;
-  if (g.D) v += g.dcoef * g.D[(int64_t)g.rowD[m] + g.colD[n]];
-  put_c(g.C + r + c, v, g.accumulate);
+  if (g.D) v += g.dcoef * ldg<IO>(g.D, (int64_t)g.rowD[m] + g.colD[n]);
+  put_c<IO>(g.C, r + c, v, g.accumulate);
   if (g.rowC2) {
     const int64_t r2 = g.rowC2[m];
-    put_c(g.C + r2 + c, -v, g.accumulate);
-    if (g.colC2) put_c(g.C + r2 + g.colC2[n], v, g.accumulate);
+    put_c<IO>(g.C, r2 + c, -v, g.accumulate);
+    if (g.colC2) put_c<IO>(g.C, r2 + g.colC2[n], v, g.accumulate);
   }
-  if (g.colC2) put_c(g.C + r + g.colC2[n], -v, g.accumulate);
+  if (g.colC2) put_c<IO>(g.C, r + g.colC2[n], -v, g.accumulate);
 }
 
+template <typename IO>
 __device__ __forceinline__ void store_c(const GemmArgs& g, int64_t m, int64_t n, double s) {
-  store_c(g, m, n, s, g.alpha);
+  store_c<IO>(g, m, n, s, g.alpha);
 }
 
 // sum_{i < cnt} q[i * stride] with four interleaved partial sums (fixed order)
@@ -174,10 +189,14 @@ struct Loader {
     }
   }
   // K offsets: k * kaff when affine, else the workgroup's copy of the table
-  // in LDS (lk[k - kbase]) when use_l, else the table in global memory
-  __device__ void fetch(double (&reg)[L], const double* __restrict__ X, const int* __restrict__ ktab,
+  // in LDS (lk[k - kbase]) when use_l, else the table in global memory.
+  // The registers hold the stored type IO: a conversion right after the
+  // load would wait for it there and expose its latency to the MFMA loop.
+  template <typename IO>
+  __device__ void fetch(IO (&reg)[L], const double* __restrict__ Xd, const int* __restrict__ ktab,
                         int64_t kaff, const int* lk, bool use_l, int64_t kbase, int64_t k0,
                         int64_t kend) const {
+    const IO* __restrict__ X = reinterpret_cast<const IO*>(Xd);
     auto ko = [&](int64_t gk) -> int64_t {
       return kaff >= 0 ? gk * kaff : (use_l ? (int64_t)lk[gk - kbase] : (int64_t)ktab[gk]);
     };
@@ -186,17 +205,17 @@ struct Loader {
       const bool kin = gk < kend;
       const int64_t o = kin ? ko(gk) : 0;
 #pragma unroll
-      for (int j = 0; j < L; ++j) reg[j] = (kin && row_off[j] >= 0) ? X[(int64_t)row_off[j] + o] : 0.0;
+      for (int j = 0; j < L; ++j) reg[j] = (kin && row_off[j] >= 0) ? X[(int64_t)row_off[j] + o] : IO(0);
     } else {
 #pragma unroll
       for (int j = 0; j < L; ++j) {
         const int64_t gk = k0 + kk[j];
-        reg[j] = (gk < kend && row_off[j] >= 0) ? X[(int64_t)row_off[j] + ko(gk)] : 0.0;
+        reg[j] = (gk < kend && row_off[j] >= 0) ? X[(int64_t)row_off[j] + ko(gk)] : IO(0);
       }
     }
   }
-  template <typename T>
-  __device__ void stash(T* lds, const double (&reg)[L]) const {
+  template <typename T, typename IO>
+  __device__ void stash(T* lds, const IO (&reg)[L]) const {
 #pragma unroll
     for (int j = 0; j < L; ++j) lds[LdsTile<T, R, KF, BK>::at(rr[j], kk[j])] = (T)reg[j];
   }
@@ -229,8 +248,9 @@ struct Mma<float> {
 // LDS-tiled GEMM C(M x N) (+)= alpha * A(M x K) B(K x N), for products with
 // both output extents large (MFMA bound).  256 threads = 4 waves in a 2 x 2
 // grid; each wave owns a (BM/2) x (BN/2) sub-tile of 16x16 MFMA tiles.
-// Operands are gathered in FP64 through int32 offset tables (rowA/kA,
-// kB/colB) and staged into LDS as T; the next K slab is fetched into
+// Operands are gathered in their stored type IO (FP64; FP32 in the
+// FP32-storage sweeps) through int32 offset tables (rowA/kA, kB/colB) and
+// staged into LDS as T; the next K slab is fetched into
 // registers while the current one is multiplied out of LDS.  T = double runs
 // on v_mfma_f64_16x16x4f64; T = float rounds the operands to FP32 and
 // accumulates in FP32 on v_mfma_f32_16x16x4f32 (the mixed-precision form).
@@ -247,7 +267,7 @@ struct Mma<float> {
 // epilogue serialises a table load, a C load and the store for every
 // element.  Outputs with mirrored images (packed pairs) take the
 // element-by-element path.
-template <int TM, int TN, typename MM, typename V>
+template <typename IO, int TM, int TN, typename MM, typename V>
 __device__ __forceinline__ void store_tile(const GemmArgs& g, int64_t m0, int64_t n0, int wm, int wn, int lane,
                                            double alpha, const V (&val)[TM][TN]) {
   if (g.rowC2 || g.colC2) {
@@ -260,7 +280,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, int64_t m0, int64_
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int64_t gm = m0 + wm + 16 * i + MM::row(lane, v);
-          if (gm < g.M) store_c(g, gm, gn, (double)val[i][j][v], alpha);
+          if (gm < g.M) store_c<IO>(g, gm, gn, (double)val[i][j][v], alpha);
         }
     }
     return;
@@ -291,7 +311,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, int64_t m0, int64_
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) dv[i][v] = rc[i][v] >= 0 ? g.D[(int64_t)rd[i][v] + cd] : 0.0;
+        for (int v = 0; v < 4; ++v) dv[i][v] = rc[i][v] >= 0 ? ldg<IO>(g.D, (int64_t)rd[i][v] + cd) : 0.0;
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -302,7 +322,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, int64_t m0, int64_
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) old[i][v] = rc[i][v] >= 0 ? g.C[(int64_t)rc[i][v] + cc] : 0.0;
+        for (int v = 0; v < 4; ++v) old[i][v] = rc[i][v] >= 0 ? ldg<IO>(g.C, (int64_t)rc[i][v] + cc) : 0.0;
 #pragma unroll
       for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -312,7 +332,7 @@ __device__ __forceinline__ void store_tile(const GemmArgs& g, int64_t m0, int64_
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int v = 0; v < 4; ++v)
-        if (rc[i][v] >= 0) g.C[(int64_t)rc[i][v] + cc] = out[i][v];
+        if (rc[i][v] >= 0) stg<IO>(g.C, (int64_t)rc[i][v] + cc, out[i][v]);
   }
 }
 
@@ -329,23 +349,23 @@ struct TabLds {
 // per slab; from global memory each is a dependent load before the data)
 constexpr int kKCache = 2048;
 
-template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+template <typename T, typename IO, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
 __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty, int tz, unsigned char* smem,
                                               int* kts);
 
-template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+template <typename T, typename IO, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
 __global__ __launch_bounds__(256 * KS, (KS == 1 && !(sizeof(T) == 8 && BM * BN >= 128 * 128)) ? 2 : 1)
 void gemm_tab(GemmArgs g) {
   __shared__ __align__(16) unsigned char smem[TabLds<T, BM, BN, BK, AKF, BKF, KS>::bytes];
   __shared__ int kts[2 * kKCache];
   int tx, ty, tz;
   tile_coords(g.group, tx, ty, tz);
-  gemm_tab_tile<T, BM, BN, BK, AKF, BKF, KS>(g, tx, ty, tz, smem, kts);
+  gemm_tab_tile<T, IO, BM, BN, BK, AKF, BKF, KS>(g, tx, ty, tz, smem, kts);
 }
 
 // The K-slab loop of the tile kernel: acc += A(m0.., kbeg..kend) B(kbeg..kend, n0..)
 // (the caller zeroes acc; group grp of KS takes slabs grp, grp + KS, ...).
-template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+template <typename T, typename IO, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
 __device__ __forceinline__ void tile_mma(const GemmArgs& g, int64_t m0, int64_t n0, int64_t kbeg, int64_t kend,
                                          unsigned char* smem, int* kts,
                                          typename Mma<T>::acc_t (&acc)[BM / 32][BN / 32]) {
@@ -367,7 +387,7 @@ __device__ __forceinline__ void tile_mma(const GemmArgs& g, int64_t m0, int64_t 
   Loader<BN, BKF, BK> lb;
   la.init(g.rowA, m0, g.M, tid);
   lb.init(g.colB, n0, g.N, tid);
-  double ra[Loader<BM, AKF, BK>::L], rb[Loader<BN, BKF, BK>::L];
+  IO ra[Loader<BM, AKF, BK>::L], rb[Loader<BN, BKF, BK>::L];
 
   // non-affine K tables of this K range into LDS (uniform condition)
   const int64_t klen = kend > kbeg ? kend - kbeg : 0;
@@ -385,8 +405,8 @@ __device__ __forceinline__ void tile_mma(const GemmArgs& g, int64_t m0, int64_t 
   const int64_t nslabs = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   const int64_t nsteps = (nslabs + KS - 1) / KS;
   if (nsteps > 0) {
-    la.fetch(ra, g.A, g.kA, g.kA_aff, kts, a_l, kbeg, kbeg + grp * BK, kend);
-    lb.fetch(rb, g.B, g.kB, g.kB_aff, kts + kKCache, b_l, kbeg, kbeg + grp * BK, kend);
+    la.template fetch<IO>(ra, g.A, g.kA, g.kA_aff, kts, a_l, kbeg, kbeg + grp * BK, kend);
+    lb.template fetch<IO>(rb, g.B, g.kB, g.kB_aff, kts + kKCache, b_l, kbeg, kbeg + grp * BK, kend);
   }
   for (int64_t s = 0; s < nsteps; ++s) {
     __syncthreads();
@@ -395,8 +415,8 @@ __device__ __forceinline__ void tile_mma(const GemmArgs& g, int64_t m0, int64_t 
     __syncthreads();
     if (s + 1 < nsteps) {
       const int64_t k0 = kbeg + ((s + 1) * KS + grp) * BK;
-      la.fetch(ra, g.A, g.kA, g.kA_aff, kts, a_l, kbeg, k0, kend);
-      lb.fetch(rb, g.B, g.kB, g.kB_aff, kts + kKCache, b_l, kbeg, k0, kend);
+      la.template fetch<IO>(ra, g.A, g.kA, g.kA_aff, kts, a_l, kbeg, k0, kend);
+      lb.template fetch<IO>(rb, g.B, g.kB, g.kB_aff, kts + kKCache, b_l, kbeg, k0, kend);
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
@@ -415,7 +435,7 @@ __device__ __forceinline__ void tile_mma(const GemmArgs& g, int64_t m0, int64_t 
   }
 }
 
-template <typename T, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
+template <typename T, typename IO, int BM, int BN, int BK, bool AKF, bool BKF, int KS>
 __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty, int tz, unsigned char* smem,
                                               int* kts) {
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -437,7 +457,7 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
   for (int i = 0; i < TM; ++i)
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = typename MM::acc_t{0, 0, 0, 0};
-  tile_mma<T, BM, BN, BK, AKF, BKF, KS>(g, m0, n0, kbeg, kend, smem, kts, acc);
+  tile_mma<T, IO, BM, BN, BK, AKF, BKF, KS>(g, m0, n0, kbeg, kend, smem, kts, acc);
 
   if (KS > 1) {
     // group 1 hands its accumulators to group 0 (slab order: group 0's sum
@@ -463,7 +483,7 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
   }
 
   if (g.nsplit == 1) {
-    store_tile<TM, TN, MM>(g, m0, n0, wm, wn, lane, g.alpha, acc);
+    store_tile<IO, TM, TN, MM>(g, m0, n0, wm, wn, lane, g.alpha, acc);
     return;
   }
 #pragma unroll
@@ -487,7 +507,7 @@ __device__ __forceinline__ void gemm_tab_tile(const GemmArgs& g, int tx, int ty,
 // times the entry's partial product to an FP64 sum, and stores once with the
 // first entry's output arguments (split z of a split-K launch: into the
 // workspace).
-template <typename T, int BM, int BN, int BK, bool AKF, bool BKF>
+template <typename T, typename IO, int BM, int BN, int BK, bool AKF, bool BKF>
 __device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1, int tx, int ty, int tz,
                                              int64_t kbeg, int64_t kend, unsigned char* smem, int* kts) {
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -521,7 +541,7 @@ __device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1,
       for (int j = 0; j < TN; ++j) acc[i][j] = typename MM::acc_t{0, 0, 0, 0};
     // the previous entry's waves are past their last LDS reads
     if (e > e0) __syncthreads();
-    tile_mma<T, BM, BN, BK, AKF, BKF, 1>(g, m0, n0, a, b, smem, kts, acc);
+    tile_mma<T, IO, BM, BN, BK, AKF, BKF, 1>(g, m0, n0, a, b, smem, kts, acc);
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -531,7 +551,7 @@ __device__ __forceinline__ void tile_entries(const GemmArgs* es, int e0, int e1,
   }
 
   if (kend < 0 || o.nsplit == 1) {
-    store_tile<TM, TN, MM>(o, m0, n0, wm, wn, lane, 1.0, sum);
+    store_tile<IO, TM, TN, MM>(o, m0, n0, wm, wn, lane, 1.0, sum);
     return;
   }
 #pragma unroll
@@ -564,7 +584,7 @@ struct AnyLds {
 // and operand layouts (C (+)= sum_e alpha_e A_e B_e (+ dcoef D)), as one
 // product over the concatenated K ranges, split along the concatenated K
 // (split z of tile (tx, ty) covers [z kchunk, (z + 1) kchunk)).
-template <typename T, int BM, int BN, int BK, bool AKF, bool BKF>
+template <typename T, typename IO, int BM, int BN, int BK, bool AKF, bool BKF>
 __global__ __launch_bounds__(256, !(sizeof(T) == 8 && BM * BN >= 128 * 128) ? 2 : 1)
 void gemm_tab_chain(GemmChain ch) {
   __shared__ __align__(16) unsigned char smem[TabLds<T, BM, BN, BK, AKF, BKF, 1>::bytes];
@@ -574,14 +594,14 @@ void gemm_tab_chain(GemmChain ch) {
   const int64_t ktot = ch.koff[ch.n];
   const int64_t kbeg = (int64_t)tz * ch.e[0].kchunk;
   const int64_t kend = (kbeg + ch.e[0].kchunk < ktot) ? kbeg + ch.e[0].kchunk : ktot;
-  tile_entries<T, BM, BN, BK, AKF, BKF>(ch.e, 0, ch.n, tx, ty, tz, kbeg, kend, smem, kts);
+  tile_entries<T, IO, BM, BN, BK, AKF, BKF>(ch.e, 0, ch.n, tx, ty, tz, kbeg, kend, smem, kts);
 }
 
 // The same tile kernel over the tiles of a group of independent GEMMs (full K
 // each).  The operand layouts (k-contiguous or not) may differ between
 // members: the workgroup branches once, uniformly, to the body of its
 // member's layouts.
-template <typename T, int BM, int BN, int BK>
+template <typename T, typename IO, int BM, int BN, int BK>
 __global__ __launch_bounds__(256, !(sizeof(T) == 8 && BM * BN >= 128 * 128) ? 2 : 1)
 void gemm_tab_group(GemmGroup gg) {
   __shared__ __align__(16) unsigned char smem[AnyLds<T, BM, BN, BK>::bytes];
@@ -590,11 +610,11 @@ void gemm_tab_group(GemmGroup gg) {
   const int mem = group_tile<BM, BN>(gg, tx, ty);
   const GemmArgs& g = gg.g[mem];
   if (g.a_kfast) {
-    if (g.b_kfast) gemm_tab_tile<T, BM, BN, BK, true, true, 1>(g, tx, ty, 0, smem, kts);
-    else gemm_tab_tile<T, BM, BN, BK, true, false, 1>(g, tx, ty, 0, smem, kts);
+    if (g.b_kfast) gemm_tab_tile<T, IO, BM, BN, BK, true, true, 1>(g, tx, ty, 0, smem, kts);
+    else gemm_tab_tile<T, IO, BM, BN, BK, true, false, 1>(g, tx, ty, 0, smem, kts);
   } else {
-    if (g.b_kfast) gemm_tab_tile<T, BM, BN, BK, false, true, 1>(g, tx, ty, 0, smem, kts);
-    else gemm_tab_tile<T, BM, BN, BK, false, false, 1>(g, tx, ty, 0, smem, kts);
+    if (g.b_kfast) gemm_tab_tile<T, IO, BM, BN, BK, false, true, 1>(g, tx, ty, 0, smem, kts);
+    else gemm_tab_tile<T, IO, BM, BN, BK, false, false, 1>(g, tx, ty, 0, smem, kts);
   }
 }
 
@@ -607,15 +627,19 @@ void gemm_tab_group(GemmGroup gg) {
 // are loaded while the current step is multiplied.
 // Workgroup = 4 waves: WK along K (same output tile, interleaved k steps,
 // combined through LDS in wave order), 4/WK along M.  Wave tile (16 RM) x (16 RN).
-template <int RM, int RN, int WK>
+// The step holds the stored type IO (converted at the MFMA, so a load in
+// flight is not waited for before the current step's MFMAs).
+template <typename IO, int RM, int RN, int WK>
 struct RegStep {
-  double a[RM][4];
-  double b[RN][4];
+  IO a[RM][4];
+  IO b[RN][4];
 };
 
-template <int RM, int RN, int WK>
-__device__ __forceinline__ void reg_load(RegStep<RM, RN, WK>& st, const GemmArgs& g, const int (&ra)[RM],
+template <typename IO, int RM, int RN, int WK>
+__device__ __forceinline__ void reg_load(RegStep<IO, RM, RN, WK>& st, const GemmArgs& g, const int (&ra)[RM],
                                          const int (&cb)[RN], int q, int64_t kb, int64_t kend) {
+  const IO* __restrict__ A = reinterpret_cast<const IO*>(g.A);
+  const IO* __restrict__ B = reinterpret_cast<const IO*>(g.B);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int64_t gk = kb + 4 * q + s;
@@ -623,24 +647,24 @@ __device__ __forceinline__ void reg_load(RegStep<RM, RN, WK>& st, const GemmArgs
     const int64_t oa = in ? koff(g.kA, g.kA_aff, gk) : 0;
     const int64_t ob = in ? koff(g.kB, g.kB_aff, gk) : 0;
 #pragma unroll
-    for (int i = 0; i < RM; ++i) st.a[i][s] = (in && ra[i] >= 0) ? g.A[(int64_t)ra[i] + oa] : 0.0;
+    for (int i = 0; i < RM; ++i) st.a[i][s] = (in && ra[i] >= 0) ? A[(int64_t)ra[i] + oa] : IO(0);
 #pragma unroll
-    for (int j = 0; j < RN; ++j) st.b[j][s] = (in && cb[j] >= 0) ? g.B[ob + cb[j]] : 0.0;
+    for (int j = 0; j < RN; ++j) st.b[j][s] = (in && cb[j] >= 0) ? B[ob + cb[j]] : IO(0);
   }
 }
 
-template <int RM, int RN, int WK>
-__device__ __forceinline__ void reg_mma(d4 (&acc)[RM][RN], const RegStep<RM, RN, WK>& st) {
+template <typename IO, int RM, int RN, int WK>
+__device__ __forceinline__ void reg_mma(d4 (&acc)[RM][RN], const RegStep<IO, RM, RN, WK>& st) {
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
     for (int i = 0; i < RM; ++i)
 #pragma unroll
       for (int j = 0; j < RN; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(st.a[i][s], st.b[j][s], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)st.a[i][s], (double)st.b[j][s], acc[i][j], 0, 0, 0);
 }
 
-template <int RM, int RN, int WK>
+template <typename IO, int RM, int RN, int WK>
 __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
   constexpr int WMW = 4 / WK;
   constexpr int TBM = WMW * 16 * RM, TBN = 16 * RN;
@@ -674,16 +698,16 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
     for (int j = 0; j < RN; ++j) acc[i][j] = d4{0, 0, 0, 0};
 
   const int64_t nsteps = kend > kbeg ? (kend - kbeg + 15) / 16 : 0;
-  RegStep<RM, RN, WK> s0, s1;
+  RegStep<IO, RM, RN, WK> s0, s1;
   int64_t st = wk;
-  if (st < nsteps) reg_load(s0, g, ra, cb, q, kbeg + 16 * st, kend);
+  if (st < nsteps) reg_load<IO>(s0, g, ra, cb, q, kbeg + 16 * st, kend);
   while (st < nsteps) {
     const int64_t n1 = st + WK;
-    if (n1 < nsteps) reg_load(s1, g, ra, cb, q, kbeg + 16 * n1, kend);
+    if (n1 < nsteps) reg_load<IO>(s1, g, ra, cb, q, kbeg + 16 * n1, kend);
     reg_mma(acc, s0);
     if (n1 >= nsteps) break;
     const int64_t n2 = n1 + WK;
-    if (n2 < nsteps) reg_load(s0, g, ra, cb, q, kbeg + 16 * n2, kend);
+    if (n2 < nsteps) reg_load<IO>(s0, g, ra, cb, q, kbeg + 16 * n2, kend);
     reg_mma(acc, s1);
     st = n2;
   }
@@ -717,7 +741,7 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
   // lane l, register v of 16x16 block (i, j) holds row 16i + l/16 + 4v and
   // column 16j + l%16: the FP64 MFMA output layout of the tile epilogue
   if (g.nsplit == 1) {
-    store_tile<RM, RN, Mma<double>>(g, wm0, n0, 0, 0, lane, g.alpha, acc);
+    store_tile<IO, RM, RN, Mma<double>>(g, wm0, n0, 0, 0, lane, g.alpha, acc);
     return;
   }
 #pragma unroll
@@ -740,6 +764,7 @@ __global__ __launch_bounds__(256) void gemm_f64_reg(GemmArgs g) {
 // order depends only on nsplit, so results are deterministic.
 // Elements are addressed as (m = blockIdx.y, n = blockIdx.x * EPB + e), so no
 // integer division is needed.
+template <typename IO>
 __global__ __launch_bounds__(256) void gemm_splitk_reduce(GemmArgs g, int pshift) {
   __shared__ double red[256];
   const int P = 1 << pshift;
@@ -758,13 +783,14 @@ __global__ __launch_bounds__(256) void gemm_splitk_reduce(GemmArgs g, int pshift
         for (int x = 1; x < P; ++x) s += red[threadIdx.x + x];
       __syncthreads();
     }
-    if (in && p == 0) store_c(g, m, n, s);
+    if (in && p == 0) store_c<IO>(g, m, n, s);
   }
 }
 
 // C(perm) (+)= alpha A through int32 offset tables: element (m, n) reads
 // A[rowA[m] + colA[n]] and updates C[rowC[m] + colC[n]].  A block covers
 // 256 >> tn_shift rows x (1 << tn_shift) columns per pass.
+template <typename IO>
 __global__ __launch_bounds__(256) void perm_add(int64_t M, int N, int tn_shift,
                                                 const double* __restrict__ A,
                                                 const int* __restrict__ rowA,
@@ -780,9 +806,8 @@ __global__ __launch_bounds__(256) void perm_add(int64_t M, int N, int tn_shift,
        m += (int64_t)gridDim.x * tm) {
     const int64_t ra = rowA[m], rc = rowC[m];
     for (int n = n0; n < N; n += tn) {
-      const double v = alpha * A[ra + colA[n]];
-      double* p = C + rc + colC[n];
-      if (accumulate) *p += v; else *p = v;
+      const double v = alpha * ldg<IO>(A, ra + colA[n]);
+      put_c<IO>(C, rc + colC[n], v, accumulate);
     }
   }
 }
@@ -802,9 +827,29 @@ __global__ __launch_bounds__(256) void gather64(int64_t M, int64_t N, const doub
 }
 
 // C (+)= alpha A over identically laid out dense tensors, two doubles per thread
+template <typename IO>
 __global__ __launch_bounds__(256) void axpy_flat(int64_t n, const double* __restrict__ A,
                                                  double* __restrict__ C, double alpha,
                                                  int accumulate) {
+  if constexpr (sizeof(IO) == 4) {
+    // four elements per thread, a grid stride apart, loaded before any store
+    const float* __restrict__ Af = reinterpret_cast<const float*>(A);
+    float* __restrict__ Cf = reinterpret_cast<float*>(C);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += 4 * stride) {
+      float a[4], c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool in = b + j * stride < n;
+        a[j] = in ? Af[b + j * stride] : 0.0f;
+        c[j] = (in && accumulate) ? Cf[b + j * stride] : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (b + j * stride < n) Cf[b + j * stride] = (float)((double)c[j] + alpha * (double)a[j]);
+    }
+    return;
+  }
   const int64_t n2 = n / 2;
   const double2* A2 = reinterpret_cast<const double2*>(A);
   double2* C2 = reinterpret_cast<double2*>(C);
@@ -830,6 +875,7 @@ __global__ __launch_bounds__(256) void axpy_flat(int64_t n, const double* __rest
 
 // Several flat C_i (+)= alpha_i A_i in one launch: element e of the
 // concatenation belongs to the member whose range holds it.
+template <typename IO>
 __global__ __launch_bounds__(256) void axpy_group(AddGroup ag) {
   const int64_t total = ag.start[ag.n];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -839,11 +885,11 @@ __global__ __launch_bounds__(256) void axpy_group(AddGroup ag) {
     for (int i = 1; i < kMaxAddGroup; ++i)
       if (i < ag.n && e >= ag.start[i]) m = i;
     const int64_t k = e - ag.start[m];
-    const double v = ag.alpha[m] * ag.A[m][k];
-    if (ag.accumulate[m]) ag.C[m][k] += v; else ag.C[m][k] = v;
+    put_c<IO>(ag.C[m], k, ag.alpha[m] * ldg<IO>(ag.A[m], k), ag.accumulate[m]);
   }
 }
 
+template <typename IO>
 __global__ __launch_bounds__(256) void perm_group(PermGroup pg) {
   const int64_t total = pg.start[pg.n];
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -854,11 +900,36 @@ __global__ __launch_bounds__(256) void perm_group(PermGroup pg) {
       if (i < pg.n && e >= pg.start[i]) m = i;
     const int64_t k = e - pg.start[m];
     const int64_t r = k / pg.N[m], c = k - r * pg.N[m];
-    const double v = pg.alpha[m] * pg.A[m][(int64_t)pg.rowA[m][r] + pg.colA[m][c]];
-    double* p = pg.C[m] + (int64_t)pg.rowC[m][r] + pg.colC[m][c];
-    if (pg.accumulate[m]) *p += v; else *p = v;
+    const double v = pg.alpha[m] * ldg<IO>(pg.A[m], (int64_t)pg.rowA[m][r] + pg.colA[m][c]);
+    put_c<IO>(pg.C[m], (int64_t)pg.rowC[m][r] + pg.colC[m][c], v, pg.accumulate[m]);
   }
 }
+
+// y = (float)x; two elements per thread (double2 -> float2) when pairs,
+// after a head element when head (x + 1 and y + 1 are the aligned starts)
+__global__ void cast_f32(float* __restrict__ y, const double* __restrict__ x, int64_t n, int pairs, int head) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (head) {
+    if (t0 == 0) y[0] = (float)x[0];
+    ++x;
+    ++y;
+    --n;
+  }
+  if (pairs) {
+    const double2* x2 = reinterpret_cast<const double2*>(x);
+    float2* y2 = reinterpret_cast<float2*>(y);
+    for (int64_t e = t0; e < n / 2; e += stride) {
+      const double2 v = x2[e];
+      y2[e] = make_float2((float)v.x, (float)v.y);
+    }
+    if (t0 == 0 && (n & 1)) y[n - 1] = (float)x[n - 1];
+    return;
+  }
+  for (int64_t e = t0; e < n; e += stride) y[e] = (float)x[e];
+}
+
+
 
 __global__ void copy_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
@@ -874,7 +945,7 @@ __global__ void fill(double* __restrict__ x, int64_t n, double v) {
 
 // One thread per element of the full block; only canonical tuples (strictly
 // increasing inside every antisymmetric group) do work.
-template <bool SUMSQ>
+template <bool SUMSQ, typename IO>
 __global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __restrict__ t,
                                                      const double* __restrict__ U,
                                                      const int* __restrict__ perms,
@@ -913,7 +984,7 @@ __global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __r
     for (int g = 0; g < P.nimg; ++g) {
       int64_t off = 0;
       for (int d = 0; d < rank; ++d) off += idx[d] * P.strides[perms[g * 8 + d]];
-      R += signs[g] * U[off];
+      R += signs[g] * ldg<IO>(U, off);
     }
     R /= (double)P.nimg;
     if (SUMSQ) sq += (double)P.nimg * R * R;
@@ -947,7 +1018,7 @@ __global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __r
 // The same step with one wave per canonical tuple, for blocks with many
 // images (a T4 block of one spin has 576): lane l gathers and scatters images
 // l, l + 64, ...; the image sum is a fixed butterfly over the wave.
-template <bool SUMSQ>
+template <bool SUMSQ, typename IO>
 __global__ __launch_bounds__(256) void jacobi_update_wave(UpdateParams P, double* __restrict__ t,
                                                           const double* __restrict__ U,
                                                           const int* __restrict__ perms,
@@ -984,7 +1055,7 @@ __global__ __launch_bounds__(256) void jacobi_update_wave(UpdateParams P, double
     for (int g = lane; g < P.nimg; g += 64) {
       int64_t off = 0;
       for (int d = 0; d < rank; ++d) off += idx[d] * P.strides[perms[g * 8 + d]];
-      R += signs[g] * U[off];
+      R += signs[g] * ldg<IO>(U, off);
     }
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) R += __shfl_xor(R, m);
@@ -1507,42 +1578,45 @@ void gemm_plan_ranked(GemmArgs& g, int64_t ws_cap, std::vector<std::pair<double,
   }
 }
 
-template <typename T, int BM, int BN, int BK, int KS>
+template <typename T, typename IO, int BM, int BN, int BK, int KS>
 static void launch_tile_k(hipStream_t s, dim3 grid, const GemmArgs& g) {
   const dim3 blk(256 * KS);
   if (g.a_kfast && g.b_kfast)
-    hipLaunchKernelGGL((gemm_tab<T, BM, BN, BK, true, true, KS>), grid, blk, 0, s, g);
+    hipLaunchKernelGGL((gemm_tab<T, IO, BM, BN, BK, true, true, KS>), grid, blk, 0, s, g);
   else if (g.a_kfast)
-    hipLaunchKernelGGL((gemm_tab<T, BM, BN, BK, true, false, KS>), grid, blk, 0, s, g);
+    hipLaunchKernelGGL((gemm_tab<T, IO, BM, BN, BK, true, false, KS>), grid, blk, 0, s, g);
   else if (g.b_kfast)
-    hipLaunchKernelGGL((gemm_tab<T, BM, BN, BK, false, true, KS>), grid, blk, 0, s, g);
+    hipLaunchKernelGGL((gemm_tab<T, IO, BM, BN, BK, false, true, KS>), grid, blk, 0, s, g);
   else
-    hipLaunchKernelGGL((gemm_tab<T, BM, BN, BK, false, false, KS>), grid, blk, 0, s, g);
+    hipLaunchKernelGGL((gemm_tab<T, IO, BM, BN, BK, false, false, KS>), grid, blk, 0, s, g);
 }
 
 // the doubled workgroup exists for the tiles up to 64x64
-template <typename T, int BM, int BN, int BK>
+template <typename T, typename IO, int BM, int BN, int BK>
 static void launch_tile_t(hipStream_t s, dim3 grid, const GemmArgs& g) {
   if constexpr (BM * BN <= 64 * 64) {
     if (g.ksplit == 2) {
-      launch_tile_k<T, BM, BN, BK, 2>(s, grid, g);
+      launch_tile_k<T, IO, BM, BN, BK, 2>(s, grid, g);
       return;
     }
   }
-  launch_tile_k<T, BM, BN, BK, 1>(s, grid, g);
+  launch_tile_k<T, IO, BM, BN, BK, 1>(s, grid, g);
 }
 
+// arithmetic / storage by GemmArgs::prec: 0 FP64 / FP64, 1 FP32 / FP64, 2 FP32 / FP32
 template <int BM, int BN, int BK>
 static void launch_tile(hipStream_t s, dim3 grid, const GemmArgs& g) {
-  if (g.prec == 1) launch_tile_t<float, BM, BN, BK>(s, grid, g);
-  else launch_tile_t<double, BM, BN, BK>(s, grid, g);
+  if (g.prec == 2) launch_tile_t<float, float, BM, BN, BK>(s, grid, g);
+  else if (g.prec == 1) launch_tile_t<float, double, BM, BN, BK>(s, grid, g);
+  else launch_tile_t<double, double, BM, BN, BK>(s, grid, g);
 }
 
 template <int RM, int RN, int WK>
 static void launch_reg(hipStream_t s, const GemmArgs& g) {
   constexpr int64_t TBM = (4 / WK) * 16 * RM, TBN = 16 * RN;
   dim3 grid((unsigned)cdiv(g.M, TBM), (unsigned)cdiv(g.N, TBN), (unsigned)g.nsplit);
-  hipLaunchKernelGGL((gemm_f64_reg<RM, RN, WK>), grid, dim3(256), 0, s, g);
+  if (g.prec == 2) hipLaunchKernelGGL((gemm_f64_reg<float, RM, RN, WK>), grid, dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((gemm_f64_reg<double, RM, RN, WK>), grid, dim3(256), 0, s, g);
 }
 
 // C^T = B^T A^T: the same product with the roles of the operands swapped.
@@ -1621,9 +1695,9 @@ static hipError_t launch_reduce(hipStream_t s, const GemmArgs& g) {
   while ((1 << pshift) < g.nsplit && (1 << pshift) < 32 && g.M * g.N * (2 << pshift) <= 262144)
     ++pshift;
   const int64_t per_block = 256 >> pshift;
-  hipLaunchKernelGGL(gemm_splitk_reduce,
-                     dim3((unsigned)cdiv(g.N, per_block), (unsigned)std::min<int64_t>(g.M, 65535)),
-                     dim3(256), 0, s, g, pshift);
+  const dim3 grid((unsigned)cdiv(g.N, per_block), (unsigned)std::min<int64_t>(g.M, 65535));
+  if (g.prec == 2) hipLaunchKernelGGL(gemm_splitk_reduce<float>, grid, dim3(256), 0, s, g, pshift);
+  else hipLaunchKernelGGL(gemm_splitk_reduce<double>, grid, dim3(256), 0, s, g, pshift);
   return hipGetLastError();
 }
 
@@ -1636,21 +1710,22 @@ hipError_t launch_gemm(hipStream_t s, const GemmArgs& g0) {
   return launch_reduce(s, g);
 }
 
-template <typename T, int BM, int BN>
+template <typename T, typename IO, int BM, int BN>
 static void launch_chain_t(hipStream_t s, dim3 grid, const GemmChain& ch) {
   const GemmArgs& o = ch.e[0];
-  if (o.a_kfast && o.b_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, BM, BN, 32, true, true>), grid, dim3(256), 0, s, ch);
-  else if (o.a_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, BM, BN, 32, true, false>), grid, dim3(256), 0, s, ch);
-  else if (o.b_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, BM, BN, 32, false, true>), grid, dim3(256), 0, s, ch);
-  else hipLaunchKernelGGL((gemm_tab_chain<T, BM, BN, 32, false, false>), grid, dim3(256), 0, s, ch);
+  if (o.a_kfast && o.b_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, IO, BM, BN, 32, true, true>), grid, dim3(256), 0, s, ch);
+  else if (o.a_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, IO, BM, BN, 32, true, false>), grid, dim3(256), 0, s, ch);
+  else if (o.b_kfast) hipLaunchKernelGGL((gemm_tab_chain<T, IO, BM, BN, 32, false, true>), grid, dim3(256), 0, s, ch);
+  else hipLaunchKernelGGL((gemm_tab_chain<T, IO, BM, BN, 32, false, false>), grid, dim3(256), 0, s, ch);
 }
 
 template <int BM, int BN>
 static void launch_chain_k(hipStream_t s, const GemmChain& ch) {
   const GemmArgs& o = ch.e[0];
   const dim3 grid((unsigned)cdiv(o.M, BM), (unsigned)cdiv(o.N, BN), (unsigned)o.nsplit);
-  if (o.prec == 1) launch_chain_t<float, BM, BN>(s, grid, ch);
-  else launch_chain_t<double, BM, BN>(s, grid, ch);
+  if (o.prec == 2) launch_chain_t<float, float, BM, BN>(s, grid, ch);
+  else if (o.prec == 1) launch_chain_t<float, double, BM, BN>(s, grid, ch);
+  else launch_chain_t<double, double, BM, BN>(s, grid, ch);
 }
 
 hipError_t launch_gemm_chain(hipStream_t s, const GemmArgs* es, int n) {
@@ -1699,8 +1774,9 @@ int64_t gemm_group_key(const GemmArgs& g0) {
 template <int BM, int BN>
 static void launch_group(hipStream_t s, const GemmGroup& gg) {
   const dim3 grid((unsigned)gg.start[gg.n]), blk(256);
-  if (gg.g[0].prec == 1) hipLaunchKernelGGL((gemm_tab_group<float, BM, BN, 32>), grid, blk, 0, s, gg);
-  else hipLaunchKernelGGL((gemm_tab_group<double, BM, BN, 32>), grid, blk, 0, s, gg);
+  if (gg.g[0].prec == 2) hipLaunchKernelGGL((gemm_tab_group<float, float, BM, BN, 32>), grid, blk, 0, s, gg);
+  else if (gg.g[0].prec == 1) hipLaunchKernelGGL((gemm_tab_group<float, double, BM, BN, 32>), grid, blk, 0, s, gg);
+  else hipLaunchKernelGGL((gemm_tab_group<double, double, BM, BN, 32>), grid, blk, 0, s, gg);
 }
 
 hipError_t launch_gemm_group(hipStream_t s, const GemmArgs* gs, int n) {
@@ -1736,24 +1812,31 @@ hipError_t launch_gemm_group(hipStream_t s, const GemmArgs* gs, int n) {
   return hipGetLastError();
 }
 
-hipError_t launch_perm_group(hipStream_t s, const PermGroup& pg) {
+hipError_t launch_perm_group(hipStream_t s, const PermGroup& pg, int io32) {
   if (pg.n < 1 || pg.n > kMaxPermGroup) return hipErrorInvalidValue;
   const int64_t total = pg.start[pg.n];
   if (total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(perm_group, dim3(grid_for(total, 256, 8192)), dim3(256), 0, s, pg);
+  const dim3 grid(grid_for(total, 256, 8192));
+  if (io32) hipLaunchKernelGGL(perm_group<float>, grid, dim3(256), 0, s, pg);
+  else hipLaunchKernelGGL(perm_group<double>, grid, dim3(256), 0, s, pg);
   return hipGetLastError();
 }
 
 hipError_t launch_perm_add(hipStream_t s, int64_t M, int64_t N, const double* A, const int* rowA,
                            const int* colA, double* C, const int* rowC, const int* colC,
-                           double alpha, int accumulate) {
+                           double alpha, int accumulate, int io32) {
   if (M <= 0 || N <= 0) return hipSuccess;
   if (N > 0x7fffffff) return hipErrorInvalidValue;
   int shift = 0;
   while ((1 << shift) < N && shift < 8) ++shift;
   const int64_t rows_per_block = 256 >> shift;
-  hipLaunchKernelGGL(perm_add, dim3(grid_for(M, (int)rows_per_block, 8192)), dim3(256), 0, s, M,
-                     (int)N, shift, A, rowA, colA, C, rowC, colC, alpha, accumulate);
+  const dim3 grid(grid_for(M, (int)rows_per_block, 8192));
+  if (io32)
+    hipLaunchKernelGGL(perm_add<float>, grid, dim3(256), 0, s, M, (int)N, shift, A, rowA, colA, C, rowC,
+                       colC, alpha, accumulate);
+  else
+    hipLaunchKernelGGL(perm_add<double>, grid, dim3(256), 0, s, M, (int)N, shift, A, rowA, colA, C, rowC,
+                       colC, alpha, accumulate);
   return hipGetLastError();
 }
 
@@ -1767,22 +1850,42 @@ hipError_t launch_gather64(hipStream_t s, int64_t M, int64_t N, const double* X,
 }
 
 hipError_t launch_axpy_flat(hipStream_t s, int64_t n, const double* A, double* C, double alpha,
-                            int accumulate) {
+                            int accumulate, int io32) {
   if (n <= 0) return hipSuccess;
+  if (io32) {
+    hipLaunchKernelGGL(axpy_flat<float>, dim3(grid_for((n + 3) / 4, 256, 8192)), dim3(256), 0, s, n, A, C,
+                       alpha, accumulate);
+    return hipGetLastError();
+  }
   // double2 access needs 16-byte alignment of both arrays
   if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(C)) & 15) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(axpy_flat, dim3(grid_for((n + 1) / 2, 256, 8192)), dim3(256), 0, s, n, A, C,
+  hipLaunchKernelGGL(axpy_flat<double>, dim3(grid_for((n + 1) / 2, 256, 8192)), dim3(256), 0, s, n, A, C,
                      alpha, accumulate);
   return hipGetLastError();
 }
 
-hipError_t launch_axpy_group(hipStream_t s, const AddGroup& ag) {
+hipError_t launch_axpy_group(hipStream_t s, const AddGroup& ag, int io32) {
   if (ag.n < 1 || ag.n > kMaxAddGroup || ag.start[0] != 0) return hipErrorInvalidValue;
   const int64_t total = ag.start[ag.n];
   if (total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(axpy_group, dim3(grid_for(total, 256, 8192)), dim3(256), 0, s, ag);
+  const dim3 grid(grid_for(total, 256, 8192));
+  if (io32) hipLaunchKernelGGL(axpy_group<float>, grid, dim3(256), 0, s, ag);
+  else hipLaunchKernelGGL(axpy_group<double>, grid, dim3(256), 0, s, ag);
   return hipGetLastError();
 }
+
+hipError_t launch_cast_f32(hipStream_t s, float* y, const double* x, int64_t n) {
+  if (n <= 0) return hipSuccess;
+  auto aligned = [](const double* a, const float* b) {
+    return ((reinterpret_cast<uintptr_t>(a) & 15) | (reinterpret_cast<uintptr_t>(b) & 7)) == 0;
+  };
+  const int head = !aligned(x, y) && n > 1 && aligned(x + 1, y + 1);
+  const int pairs = aligned(x + head, y + head);
+  hipLaunchKernelGGL(cast_f32, dim3(grid_for(pairs ? (n + 1) / 2 : n, 256, 8192)), dim3(256), 0, s, y, x,
+                     n, pairs, head);
+  return hipGetLastError();
+}
+
 
 // MFMA issue-rate probe (the attainable matrix-core rate on this part, the
 // ceiling the contraction kernels are compared with): every wave runs `iters`
@@ -1852,26 +1955,31 @@ int update_blocks(const UpdateParams& P) {
   return P.nimg >= kWaveImages ? grid_for(P.total * 64, 256, 16384) : grid_for(P.total, 256);
 }
 
+template <typename IO>
+static void launch_update_io(hipStream_t s, const UpdateParams& P, double* t, const double* U,
+                             const int* perms, const double* signs, const double* fa,
+                             const double* fb, double* sumsq) {
+  const dim3 grid(update_blocks(P)), blk(256);
+  if (P.nimg >= kWaveImages) {
+    if (sumsq)
+      hipLaunchKernelGGL((jacobi_update_wave<true, IO>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
+    else
+      hipLaunchKernelGGL((jacobi_update_wave<false, IO>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
+    return;
+  }
+  if (sumsq)
+    hipLaunchKernelGGL((jacobi_update<true, IO>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
+  else
+    hipLaunchKernelGGL((jacobi_update<false, IO>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
+}
+
 hipError_t launch_update(hipStream_t s, const UpdateParams& P, double* t, const double* U,
                          const int* perms, const double* signs, const double* fa,
                          const double* fb, double* sumsq) {
   if (!P.coc || !P.cvc || P.n_vc <= 0) return hipErrorInvalidValue;
   if (P.total <= 0) return hipSuccess;
-  if (P.nimg >= kWaveImages) {
-    if (sumsq)
-      hipLaunchKernelGGL(jacobi_update_wave<true>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U,
-                         perms, signs, fa, fb, sumsq);
-    else
-      hipLaunchKernelGGL(jacobi_update_wave<false>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U,
-                         perms, signs, fa, fb, sumsq);
-    return hipGetLastError();
-  }
-  if (sumsq)
-    hipLaunchKernelGGL(jacobi_update<true>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U, perms,
-                       signs, fa, fb, sumsq);
-  else
-    hipLaunchKernelGGL(jacobi_update<false>, dim3(update_blocks(P)), dim3(256), 0, s, P, t, U,
-                       perms, signs, fa, fb, sumsq);
+  if (P.u32) launch_update_io<float>(s, P, t, U, perms, signs, fa, fb, sumsq);
+  else launch_update_io<double>(s, P, t, U, perms, signs, fa, fb, sumsq);
   return hipGetLastError();
 }
 

@@ -229,7 +229,9 @@ class Engine:
         return X, M, Y
 
     def set_precision(self, mode: int):
-        """0: FP64 MFMA contractions; 1: FP32 MFMA for the tiled contractions."""
+        """0: FP64 MFMA contractions; 1: FP32 MFMA for the tiled contractions;
+        2: as 1, with every tensor of the sweep stored in FP32 as well (the
+        updated amplitudes are FP64 in every mode)."""
         self._check(self.lib.ccq_hip_set_precision(self.h, int(mode)), "set_precision")
 
     # CC(t;3) tail (calc_type "CC(t;3)")

@@ -57,9 +57,12 @@ class Mixed:
     amplitudes are those of the FP64 equations.  The default switch lies
     above the FP32-contraction noise of dE (1e-6 - 1e-5 on the 40-orbital
     CCSDT and 100-orbital CCSD systems), where the FP32 phase leaves the FP64
-    trajectory's iteration count unchanged."""
+    trajectory's iteration count unchanged.  mode: the precision mode of the
+    FP32 sweeps (1: FP32 MFMA on FP64 storage; 2: FP32 storage of every
+    tensor of the sweep as well, fewer bytes per sweep)."""
     switch_de: float = 3e-5
     refine_every: int = 0
+    mode: int = 1
 
 
 def restart(eng: Engine, path: str = "t_vec.bin", bin_file: str | None = None,
@@ -93,7 +96,7 @@ def jacobi_iter(eng: Engine, tol: float, max_iter: int, diis_space: int,
     for it in range(1, max_iter + 1):
         if mixed is not None:
             fp32 = low and not (mixed.refine_every and it % mixed.refine_every == 0)
-            eng.set_precision(1 if fp32 else 0)
+            eng.set_precision(mixed.mode if fp32 else 0)
         eng.update()
         e_new = eng.energy()
         de = en_cor - e_new

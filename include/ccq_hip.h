@@ -225,9 +225,12 @@ const char* ccq_hip_mlp_last_error(void);
 /* Arithmetic of the bulk contractions of later ccq_hip_update calls: 0 FP64
  * MFMA (default, ccq's precision); 1 FP32 MFMA for the LDS-tiled
  * contractions (operands rounded to FP32 on the way into LDS, FP32
- * accumulation, added into FP64 results).  Denominators, updates, energy,
- * DIIS and the convergence test stay FP64; the caller refines with FP64
- * sweeps (mode 0) before it stops.  No counterpart in ccq (FP64 only). */
+ * accumulation, added into FP64 results); 2 as 1, with the intermediates,
+ * residuals, integral views and a copy of the amplitudes of the sweep held
+ * in FP32 (half the bytes per sweep; float shadows allocated at the first
+ * such sweep).  The updated amplitudes, denominators, energy, DIIS and the
+ * convergence test stay FP64; the caller refines with FP64 sweeps (mode 0)
+ * before it stops.  No counterpart in ccq (FP64 only). */
 int ccq_hip_set_precision(ccq_hip_ctx* ctx, int64_t mode);
 
 /* Optional per-operation timing with HIP events on the context's stream.

@@ -1,7 +1,7 @@
 """The captured sweep graph gives bit-for-bit the amplitudes of eager
 launches, replay after replay, with eager work and synchronisations in
-between -- for CCSD, CCSDT (FP64 and FP32 contractions), CCSDTQ, active-space
-CCSDt and the left-CC program of CC(t;3)."""
+between -- for CCSD, CCSDT (FP64, FP32 contractions, FP32 storage), CCSDTQ,
+active-space CCSDt and the left-CC program of CC(t;3)."""
 
 import sys
 
@@ -41,8 +41,9 @@ def _engines(c, method, monkeypatch, **kw):
 
 
 @pytest.mark.parametrize("method,n,o,ob,prec", [("CCSD", 16, 4, 3, 0), ("CCSDT", 9, 3, 2, 0),
-                                                ("CCSDT", 9, 3, 2, 1), ("CCSDTQ", 8, 3, 2, 0),
-                                                ("CCSDt", 9, 3, 2, 0)])
+                                                ("CCSDT", 9, 3, 2, 1), ("CCSDT", 9, 3, 2, 2),
+                                                ("CCSDTQ", 8, 3, 2, 0), ("CCSDTQ", 8, 3, 2, 2),
+                                                ("CCSDt", 9, 3, 2, 0), ("CCSDt", 9, 3, 2, 2)])
 def test_graph_equals_eager(method, n, o, ob, prec, monkeypatch):
     _say(f"case graph {method} n={n} prec={prec}")
     c = Case(n, o, 2 * n, 31, occ_b=ob)

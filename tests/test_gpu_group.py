@@ -228,18 +228,20 @@ def test_switch_off_matches_default(switch, method, n, o, ob, monkeypatch):
     assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
 
 
-def test_fp32_grouped_and_chained_close_to_fp64(monkeypatch):
-    """The FP32 forms of the grouped and chained launches (mixed precision):
-    one FP32 sweep at a size where both form stays within 1e-5 (relative to
-    the step) of the FP64 sweep and differs from it."""
-    _say("case fp32 grouped and chained")
+@pytest.mark.parametrize("prec", [1, 2])
+def test_fp32_grouped_and_chained_close_to_fp64(prec, monkeypatch):
+    """The FP32 forms of the grouped and chained launches (mixed precision,
+    FP32 contractions on FP64 or FP32 storage): one FP32 sweep at a size
+    where both forms run stays within 1e-5 (relative to the step) of the
+    FP64 sweep and differs from it."""
+    _say(f"case fp32 grouped and chained, precision {prec}")
     c = Case(64, 8, 128, 47)
     vec = _state(c, "CCSD", 17)
     out = []
-    for prec in (0, 1):
+    for p in (0, prec):
         eng = _engine_env(c, "CCSD", monkeypatch, {"CCQ_HIP_GRAPH": "0"}, shift=0.02)
         eng.set_t(vec)
-        eng.set_precision(prec)
+        eng.set_precision(p)
         eng.profile(True)
         eng.update()
         rep = eng.profile_report()
