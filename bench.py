@@ -21,8 +21,10 @@ Protocol (per workload)
      1e-7 of E_conv;
   4. learned leg: the learned starting guess (computed inside the timed
      region; MP2 if no weights file), then FP64 iterations as above;
-  5. mixed+learned leg: the learned guess, then FP32-MFMA contractions until
-     |dE| < 3e-5 (above the FP32 noise of dE on these systems), FP64 after
+  5. mixed+learned leg: the learned guess, then FP32 sweeps (FP32-MFMA
+     contractions on FP32 copies of every tensor of the sweep; the
+     amplitudes themselves stay FP64) until |dE| < 3e-5 (above the FP32
+     noise of dE on these systems), FP64 after
      that; the iteration of the switch is recorded and, as in solve.Mixed,
      only FP64 sweeps count as converged;
   6. value = the mixed+learned leg's wall time until the first FP64
@@ -63,6 +65,7 @@ from ccq_amd.inputs import build_integrals  # noqa: E402
 METRIC = "time-to-|dE|<1e-7 Eh, mixed+learned vs plain FP64 (1 GPU); % FP64 MFMA peak"
 PEAK_FP64_MFMA = 78.6e12
 SWITCH_DE = 3e-5     # mixed leg: FP32 contractions until |dE| falls below this
+MIXED_MODE = 2       # precision mode of the FP32 sweeps: FP32 MFMA on FP32 storage
 HIT = 1e-7
 MAX_ITER = 200       # a leg that has not reached HIT after K iterations continues up to this
 
@@ -94,7 +97,7 @@ class Leg:
         energies, stamps, fp64 = [], [], []
         low = switch_de is not None
         switch_it = None
-        eng.set_precision(1 if low else 0)
+        eng.set_precision(MIXED_MODE if low else 0)
         en_cor = 0.0
         eng.sync()
         start = time.perf_counter()
@@ -392,7 +395,8 @@ def main():
         return out
 
     cfg = summary(main_r)
-    cfg["mode"] = (f"mixed+learned: {main_r['guess']} starting guess, FP32-MFMA contractions until "
+    cfg["mode"] = (f"mixed+learned: {main_r['guess']} starting guess, FP32 sweeps (precision mode "
+                   f"{MIXED_MODE}: FP32-MFMA contractions on FP32 storage) until "
                    f"|dE|<{SWITCH_DE:g}, then FP64; only FP64 sweeps count as converged")
     if other:
         cfg["uhf" if rhf else "rhf"] = summary(other)

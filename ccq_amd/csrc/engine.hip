@@ -1997,13 +1997,31 @@ struct ccq_hip_ctx {
             grouped.insert(members.begin(), members.end());
           }
         }
+        // the permuted adds of this level (their own tables each) as
+        // perm_group launches; adds into the amplitudes stay alone
+        {
+          std::vector<int> adds;
+          for (int i : here) {
+            const Op& op = sec.ops[i];
+            if (!op.gemm && !op.flat && op.part_of < 0 && !grouped.count(i) &&
+                tensors[op.out].kind != TKind::Amp)
+              adds.push_back(i);
+          }
+          for (size_t lo = 0; lo + 1 < adds.size(); lo += ccqk::kMaxPermGroup) {
+            const size_t hi = std::min(adds.size(), lo + (size_t)ccqk::kMaxPermGroup);
+            if (hi - lo < 2) break;
+            std::vector<int> members(adds.begin() + lo, adds.begin() + hi);
+            group_at[members[0]] = members;
+            grouped.insert(members.begin(), members.end());
+          }
+        }
         // this level in op order, each group where its first member was
         for (int i : here) {
           auto it = group_at.find(i);
           if (it != group_at.end()) {
             sec.launches.push_back(it->second);
             sec.launch_cls.push_back(sec.ops[i].gemm ? group_cls(sec, it->second)
-                                     : (sec.ops[i].part_of >= 0 ? "add:perm*" : "add:flat*") +
+                                     : (sec.ops[i].part_of >= 0 || !sec.ops[i].flat ? "add:perm*" : "add:flat*") +
                                            std::to_string(it->second.size()));
             sec.launch_level.push_back(lv);
           } else if (!grouped.count(i)) {
@@ -2459,7 +2477,7 @@ struct ccq_hip_ctx {
             for (int i : L) fl += sec.ops[i].flops, by += sec.ops[i].bytes;
             if (sec.ops[L[0]].gemm)
               timed(sec.launch_cls[k], fl, by, [&] { run_group(sec, L); });
-            else if (sec.ops[L[0]].part_of >= 0)
+            else if (sec.ops[L[0]].part_of >= 0 || !sec.ops[L[0]].flat)
               timed(sec.launch_cls[k], fl, by, [&] { run_perm_group(sec, L); });
             else
               timed(sec.launch_cls[k], fl, by, [&] { run_add_group(sec, L); });
