@@ -16,7 +16,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libccq_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("engine.hip", "kernels.hip", "guess.hip")]
+# the GEMM kernel instantiations are split by arithmetic / storage pair
+# (gemm_f64, gemm_f32, gemm_f32io) so that they compile in parallel
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("engine.hip", "kernels.hip", "guess.hip", "gemm_f64.hip",
+                                                    "gemm_f32.hip", "gemm_f32io.hip")]
 # host-only sources (no device code): built with the host compiler
 HOST_SOURCES = [os.path.join(HERE, "csrc", f) for f in ("ints_io.cpp", "restart.cpp")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
