@@ -151,8 +151,36 @@ __device__ inline void load_weights(float* w, const float* __restrict__ params) 
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void guess_apply(GuessBlock B, const float* __restrict__ params) {
+// Inference of one sample per thread, the same operations in the same order
+// as mlp_forward, with the first hidden layer in LDS (hs: this thread's
+// column, stride kApplyThreads) instead of a per-thread array, which the
+// run-time indexed loops would otherwise place in scratch memory.
+constexpr int kApplyThreads = 128;
+__device__ inline float mlp_infer(const float* __restrict__ w, const float (&xin)[F], float* hs) {
+  float x[F];
+#pragma unroll
+  for (int k = 0; k < F; ++k) x[k] = bf16r((xin[k] - w[P_MU + k]) / w[P_SD + k]);
+#pragma unroll 8
+  for (int p = 0; p < H; ++p) {
+    float acc = w[P_B1 + p];
+#pragma unroll
+    for (int k = 0; k < F; ++k) acc += w[P_W1 + k * H + p] * x[k];
+    hs[p * kApplyThreads] = bf16r(tanhf(acc));
+  }
+  float out = w[P_B3];
+  for (int q = 0; q < H; ++q) {
+    float acc = w[P_B2 + q];
+#pragma unroll 16
+    for (int p = 0; p < H; ++p) acc += w[P_W2 + p * H + q] * hs[p * kApplyThreads];
+    out += w[P_W3 + q] * bf16r(tanhf(acc));
+  }
+  return out;
+}
+
+__global__ __launch_bounds__(kApplyThreads) void guess_apply(GuessBlock B, const float* __restrict__ params) {
   __shared__ float w[kGuessParams];
+  __shared__ float h1s[H * kApplyThreads];
+  float* hs = h1s + threadIdx.x;
   if (params) load_weights(w, params);
   if (B.anti) {
     // t[ijab] = -t[jiab] = -t[ijba] = t[jiba]; the network sees features
@@ -166,7 +194,7 @@ __global__ __launch_bounds__(256) void guess_apply(GuessBlock B, const float* __
       pair_of(e % pv, B.v1, a, b);
       const Tuple t = tuple_at(B, i, j, a, b);
       double v = t.m;
-      if (params) v = t.m * (1.0 + (double)mlp_forward(w, t.x, nullptr));
+      if (params) v = t.m * (1.0 + (double)mlp_infer(w, t.x, hs));
       const int64_t o = B.o2, vv = B.v2;
       B.t[((i * o + j) * B.v1 + a) * vv + b] = v;
       B.t[((j * o + i) * B.v1 + a) * vv + b] = -v;
@@ -181,7 +209,7 @@ __global__ __launch_bounds__(256) void guess_apply(GuessBlock B, const float* __
     int64_t off;
     const Tuple t = guess_tuple(B, e, off);
     double v = t.m;
-    if (params) v = t.m * (1.0 + (double)mlp_forward(w, t.x, nullptr));
+    if (params) v = t.m * (1.0 + (double)mlp_infer(w, t.x, hs));
     B.t[off] = v;
   }
 }
@@ -216,7 +244,9 @@ hipError_t launch_guess_apply(hipStream_t s, const GuessBlock& B, const float* p
   const int64_t total = B.anti ? (B.o1 * (B.o1 - 1) / 2) * (B.v1 * (B.v1 - 1) / 2)
                                : B.o1 * B.o2 * B.v1 * B.v2;
   if (total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(guess_apply, dim3(grid_of(total)), dim3(256), 0, s, B, params);
+  const int64_t blocks = (total + kApplyThreads - 1) / kApplyThreads;
+  hipLaunchKernelGGL(guess_apply, dim3((unsigned)(blocks < 8192 ? blocks : 8192)), dim3(kApplyThreads), 0, s, B,
+                     params);
   return hipGetLastError();
 }
 
