@@ -2508,9 +2508,15 @@ struct ccq_hip_ctx {
     }
   }
 
-  double reduce_partials(int count) {
+  // diis_record of the right amplitudes also computes the next residuum's
+  // partial sums (rpart); a residuum of the same iteration that is the very
+  // next call of the C ABI uses them (api_calls counts the calls)
+  int64_t api_calls = 0, rec_call = -10, rec_iter = -1;
+  DevBuf rpart;
+
+  double reduce_partials(int count, const double* src = nullptr) {
     host_partial.resize(count);
-    HIPCHK(hipMemcpyAsync(host_partial.data(), partial.p, sizeof(double) * count,
+    HIPCHK(hipMemcpyAsync(host_partial.data(), src ? src : partial.p, sizeof(double) * count,
                           hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     double s = 0.0;
@@ -2944,6 +2950,7 @@ bool solve_dense(std::vector<double> A, std::vector<double>& b, int n) {
 template <class F>
 int guarded(ccq_hip_ctx* ctx, F&& fn) {
   if (!ctx) return 2;
+  ++ctx->api_calls;
   try {
     fn();
     return 0;
@@ -3281,13 +3288,21 @@ int ccq_hip_diis_record(ccq_hip_ctx* ctx, int64_t iter) {
       ctx->pack(ctx->ring.p + s * ctx->vec_size);
       return;
     }
-    HIPCHK(hipMemcpyAsync(ctx->ring.p + s * ctx->vec_size, ctx->tvec.p,
-                          sizeof(double) * ctx->vec_size, hipMemcpyDeviceToDevice, ctx->stream));
+    const double* prev = iter == 1 ? nullptr : ctx->ring.p + ctx->slot_of(iter - 1) * ctx->vec_size;
+    if (!ctx->rpart.p) ctx->rpart.alloc(kPartialBlocks);
+    HIPCHK(ccqk::launch_copy_diff_sq(ctx->stream, ctx->ring.p + s * ctx->vec_size, ctx->tvec.p, prev,
+                                     ctx->vec_size, ctx->rpart.p, kPartialBlocks));
+    ctx->rec_call = ctx->api_calls;
+    ctx->rec_iter = iter;
   });
 }
 
 int ccq_hip_residuum(ccq_hip_ctx* ctx, int64_t iter, double* res) {
   return guarded(ctx, [&] {
+    if (!ctx->left_mode && !ctx->active && ctx->rec_iter == iter && ctx->rec_call == ctx->api_calls - 1) {
+      *res = std::sqrt(std::sqrt(ctx->reduce_partials(kPartialBlocks, ctx->rpart.p)));
+      return;
+    }
     const int64_t len = ctx->diis_len();
     const double* cur = ctx->ring.p + ctx->slot_of(iter) * len;
     const double* prev = iter == 1 ? nullptr : ctx->ring.p + ctx->slot_of(iter - 1) * len;

@@ -272,6 +272,9 @@ hipError_t launch_dot_strided(hipStream_t s, int64_t rows, int64_t cols, const d
                               int64_t lda, const double* b, double* partial, int blocks);
 hipError_t launch_diis_gram(hipStream_t s, const double* ring, int64_t len, int n, double* partial,
                             int blocks);
+// dst = src with diff_sq(prev, dst)'s partials in the same pass
+hipError_t launch_copy_diff_sq(hipStream_t s, double* dst, const double* src, const double* prev,
+                               int64_t len, double* partial, int blocks);
 hipError_t launch_diff_sq(hipStream_t s, const double* a, const double* b, int64_t len,
                           double* partial, int blocks);
 hipError_t launch_lincomb(hipStream_t s, double* out, const double* ring, int64_t len, int n,

@@ -635,6 +635,31 @@ __global__ __launch_bounds__(256) void diff_sq(const double* __restrict__ a, con
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// diis_record and the next residuum in one pass: dst = src, and the block
+// partials of sum (prev - src)^2 (prev == nullptr: sum src^2) with the
+// element-to-thread map and reduction of diff_sq, so the partials equal
+// diff_sq(prev, dst)'s
+__global__ __launch_bounds__(256) void copy_diff_sq(double* __restrict__ dst, const double* __restrict__ src,
+                                                    const double* __restrict__ prev, int64_t len,
+                                                    double* __restrict__ partial) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < len;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const double v = src[e];
+    dst[e] = v;
+    const double d = (prev ? prev[e] : 0.0) - v;
+    acc += d * d;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
 __global__ void lincomb(double* __restrict__ out, const double* __restrict__ ring, int64_t len,
                         int n, Coeffs c) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < len;
@@ -1199,6 +1224,12 @@ hipError_t launch_diis_gram(hipStream_t s, const double* ring, int64_t len, int 
 hipError_t launch_diff_sq(hipStream_t s, const double* a, const double* b, int64_t len,
                           double* partial, int blocks) {
   hipLaunchKernelGGL(diff_sq, dim3(blocks), dim3(256), 0, s, a, b, len, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_diff_sq(hipStream_t s, double* dst, const double* src, const double* prev,
+                               int64_t len, double* partial, int blocks) {
+  hipLaunchKernelGGL(copy_diff_sq, dim3(blocks), dim3(256), 0, s, dst, src, prev, len, partial);
   return hipGetLastError();
 }
 
