@@ -114,6 +114,10 @@ struct Op {
   int part_of = -1;
   bool has_gplan = false;
   ccqk::GemmArgs gplan{};
+  // plan measured in FP32 storage (precision mode 2) for a product of the
+  // sweep launched on its own; grouped and chained launches keep their plans
+  bool has_plan32 = false;
+  ccqk::GemmArgs plan32{};
   double dcoef = 0.0;
   int64_t t_rowD = 0, t_colD = 0;
   // antisymmetric pairs packed to their canonical half: a summed pair (k),
@@ -1194,8 +1198,10 @@ struct ccq_hip_ctx {
     dst.ksplit = src.ksplit;
   }
 
-  void autotune(const std::vector<int>& host32) {
-    run_prec = 0;   // candidates are timed in FP64 on the FP64 tensors
+  // prec 0: plans timed in FP64 on the FP64 tensors (op.plan); prec 2: the
+  // sweep's products timed in FP32 storage on the float shadows (op.plan32)
+  void autotune(const std::vector<int>& host32, int prec = 0) {
+    run_prec = prec;
     int* dt = nullptr;
     HIPCHK(hipMalloc(&dt, sizeof(int) * std::max<size_t>(host32.size(), 1)));
     HIPCHK(hipMemcpy(dt, host32.data(), sizeof(int) * host32.size(), hipMemcpyHostToDevice));
@@ -1208,6 +1214,7 @@ struct ccq_hip_ctx {
     auto restore = [&] {
       tables32 = saved_tab;
       cur_ws = nullptr;
+      run_prec = 0;
       if (e0) (void)hipEventDestroy(e0);
       if (e1) (void)hipEventDestroy(e1);
       (void)hipFree(dt);
@@ -1217,15 +1224,20 @@ struct ccq_hip_ctx {
       HIPCHK(hipEventCreate(&e1));
       TuneCache& cache = tune_cache();
       for (auto& sec : sections) {
-        if (sec.kind != Section::Compute) continue;
+        if (sec.kind != Section::Compute || (prec == 2 && sec.prog != "sweep")) continue;
         for (auto& op : sec.ops) {
           if (!op.gemm || op.has_gplan || op.K == 0) continue;
-          const std::string key = tune_key(op);
+          const std::string key = (prec == 2 ? "p2:" : "") + tune_key(op);
+          if (prec == 2) {
+            op.plan32 = op.plan;
+            op.has_plan32 = true;
+          }
+          ccqk::GemmArgs& dst = prec == 2 ? op.plan32 : op.plan;
           {
             std::lock_guard<std::mutex> lk(cache.mu);
             auto hit = cache.plans.find(key);
             if (hit != cache.plans.end()) {
-              copy_plan(op.plan, hit->second);
+              copy_plan(dst, hit->second);
               continue;
             }
           }
@@ -1244,10 +1256,11 @@ struct ccq_hip_ctx {
             if ((int)cands.size() == kTuneCandidates) break;
           }
           double best_t = 1e300;
-          ccqk::GemmArgs best = op.plan;
+          ccqk::GemmArgs best = dst;
           for (auto& c : cands) {
             Op trial = op;
             copy_plan(trial.plan, c);
+            trial.has_plan32 = false;
             ccqk::GemmArgs a = gemm_args(trial);
             HIPCHK(ccqk::launch_gemm(stream, a));
             HIPCHK(hipEventRecord(e0, stream));
@@ -1261,7 +1274,7 @@ struct ccq_hip_ctx {
               best = c;
             }
           }
-          copy_plan(op.plan, best);
+          copy_plan(dst, best);
           std::lock_guard<std::mutex> lk(cache.mu);
           cache.plans.emplace(key, best);
         }
@@ -1568,10 +1581,17 @@ struct ccq_hip_ctx {
     }
     if (tune) {
       autotune(host32);
+      if (tune32) {
+        prep_f32();
+        autotune(host32, 2);
+      }
       ws_need = 0;
       for (auto& sec : sections)
         for (auto& op : sec.ops)
-          if (op.gemm) ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(op.plan));
+          if (op.gemm) {
+            ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(op.plan));
+            if (op.has_plan32) ws_need = std::max<int64_t>(ws_need, ccqk::gemm_ws_need(op.plan32));
+          }
     }
     fuse_adds(push32);
     // groups first; products left alone are chained where they can be
@@ -1805,8 +1825,12 @@ struct ccq_hip_ctx {
   bool group = true;
   // CCSDt triples residuals on the P space only (CCQ_HIP_PSPACE=0: full blocks)
   bool pspace = true;
-  // measuring autotuner over the planner's best candidate plans (CCQ_HIP_TUNE=1)
+  // measuring autotuner over the planner's best candidate plans (CCQ_HIP_TUNE=1);
+  // with it, the sweep's products are also timed in FP32 storage, whose
+  // float shadows it then allocates (CCQ_HIP_TUNE32=0: FP32 sweeps use the
+  // FP64 plans)
   bool tune = false;
+  bool tune32 = true;
   // point-group blocking: 0-based irrep of every correlated orbital (abelian
   // groups up to D2h, products by XOR); every tensor of the programs is
   // totally symmetric, so a product C = A B splits into the blocks of rows,
@@ -2069,7 +2093,7 @@ struct ccq_hip_ctx {
   // execution
 
   ccqk::GemmArgs gemm_args(const Op& op) const {
-    ccqk::GemmArgs g = op.plan;
+    ccqk::GemmArgs g = run_prec == 2 && op.has_plan32 ? op.plan32 : op.plan;
     g.A = sp(op.a);
     g.rowA = tables32 + op.t_rowA;
     g.kA = tables32 + op.t_kA;
@@ -3019,6 +3043,7 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (const char* g = std::getenv("CCQ_HIP_SCHEDULE")) ctx->dump_schedule = std::string(g) == "1";
     if (const char* g = std::getenv("CCQ_HIP_PSPACE")) ctx->pspace = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_TUNE")) ctx->tune = std::string(g) == "1";
+    if (const char* g = std::getenv("CCQ_HIP_TUNE32")) ctx->tune32 = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_GROUP_MINK")) ctx->group_min_k = std::max(1, std::atoi(g));
     if (const char* g = std::getenv("CCQ_HIP_GROUP_REG")) ctx->group_reg = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_STREAMS"))

@@ -368,3 +368,39 @@ def test_autotuned_plans_match_model_plans(method, n, o, ob, monkeypatch):
     assert np.abs(t1 - t0).max() <= 1e-11 * np.abs(t0).max()
     np.testing.assert_array_equal(e2, e1)
     np.testing.assert_array_equal(t2, t1)
+
+
+@pytest.mark.parametrize("method,n,o,ob,rhf", [("CCSD", 24, 5, 5, True), ("CCSDT", 12, 3, 3, True),
+                                               ("CCSDT", 12, 3, 2, False)])
+def test_fp32_storage_autotuned_plans(method, n, o, ob, rhf, monkeypatch):
+    """With the autotuner on, the sweep's products also get plans measured in
+    FP32 storage (precision mode 2): three mode-2 sweeps with those plans
+    agree with mode-2 sweeps on the FP64 plans (CCQ_HIP_TUNE32=0) to FP32
+    rounding (1e-6 of the largest amplitude) and stay within 1e-5 of the
+    FP64 sweeps' change of the amplitudes."""
+    _say(f"case fp32-storage tuned plans {method} n={n} o={o} ob={ob} rhf={rhf}")
+    c = Case(n, o, 2 * n, 67, occ_b=ob)
+    vec = _state(c, method, 31)
+    if rhf:
+        t = E.split_tvec(vec, c.ca, c.cb, c.va, c.vb)
+        t["t1b"] = t["t1a"].copy()
+        t["t2bb"] = t["t2aa"].copy()
+        if method == "CCSDT":
+            t["t3bbb"] = t["t3aaa"].copy()
+            t["t3abb"] = osolver.tran3bto3c(t["t3aab"])
+        vec = E.join_tvec(t, c.ca, c.cb, c.va, c.vb)
+    out = []
+    for prec, tune32 in ((0, "1"), (2, "1"), (2, "0")):
+        eng = _engine_env(c, method, monkeypatch, {"CCQ_HIP_TUNE": "1", "CCQ_HIP_TUNE32": tune32,
+                                                   "CCQ_HIP_GRAPH": "1"}, shift=0.02, rhf=rhf)
+        eng.set_t(vec)
+        eng.set_precision(prec)
+        for _ in range(3):
+            eng.update()
+        out.append(eng.get_t())
+        eng.close()
+    t64, t32, t32m = out
+    scale = np.abs(t64).max()
+    step = np.abs(t64 - vec).max()
+    assert np.abs(t32 - t32m).max() <= 1e-6 * scale
+    assert 1e-13 < np.abs(t32 - t64).max() < 1e-5 * step
