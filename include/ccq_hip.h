@@ -241,7 +241,9 @@ int ccq_hip_profile_report(ccq_hip_ctx* ctx, char* buf, int64_t buflen);
 
 /* Copy one dense tensor of the loaded program (an intermediate, residual
  * accumulator or amplitude block, by its IR id) to the host.  Inspection
- * only; ccq has no counterpart. */
+ * only; ccq has no counterpart.  Reads the FP64 storage: after a precision
+ * mode 2 sweep, intermediates and residuals hold the last FP64 / mode-1
+ * sweep's values (amplitude blocks are always current). */
 int ccq_hip_debug_read(ccq_hip_ctx* ctx, const char* tensor_id, double* out, int64_t n);
 /* Run only the first nsections sections of one sweep (inspection only). */
 int ccq_hip_debug_sweep(ccq_hip_ctx* ctx, int64_t nsections);
