@@ -56,6 +56,27 @@ constexpr int64_t kWsCap = 16 << 20;   // split-K workspace bound (doubles)
 // ---------------------------------------------------------------------------
 // device buffers
 
+// Pinned host array for the small device-to-host copies of the partial sums
+// (energy, residuum, DIIS Gram matrix): a copy into pageable memory goes
+// through a staging buffer and costs several times as long.
+struct PinnedVec {
+  double* p = nullptr;
+  size_t cap = 0;
+  void resize(size_t n) {
+    if (n <= cap) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(double) * n, hipHostMallocDefault));
+    cap = n;
+  }
+  double* data() { return p; }
+  double& operator[](size_t i) { return p[i]; }
+  ~PinnedVec() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 struct DevBuf {
   double* p = nullptr;
   int64_t n = 0;
@@ -267,7 +288,7 @@ struct ccq_hip_ctx {
   // profiling
   bool profile = false;
   std::map<std::string, ProfEntry> prof;
-  std::vector<double> host_partial;
+  PinnedVec host_partial;
 
   ~ccq_hip_ctx() {
     for (auto& kv : graphs) drop_graph(kv.second);
