@@ -269,6 +269,11 @@ struct ccq_hip_ctx {
   // as mode 1.
   int prec = 0;
   int run_prec = 0;          // mode of the program being run
+  // the float copy of t_vec equals t_vec: set by a mode-2 sweep (which casts
+  // every block it updates or copies), cleared on entry to every C-ABI call
+  // that may write t_vec (all but the read-only ones, see guarded); a mode-2
+  // sweep casts t_vec first only when it was not valid at its call
+  bool t32_valid = false, t32_valid_at_call = false;
   bool f32_ready = false;
   bool views32_stale = true;   // views gathered again since their last cast
   float* tvec32 = nullptr;
@@ -2366,8 +2371,15 @@ struct ccq_hip_ctx {
     G.g = nullptr;
   }
 
+  // float shadows ready, and the float copy of t_vec refreshed unless it
+  // was still valid at this call (cast on the stream, outside any graph)
+  void prep_sweep32() {
+    prep_f32();
+    if (!t32_valid_at_call) HIPCHK(ccqk::launch_cast_f32(stream, tvec32, tvec.p, dev_t));
+  }
+
   void run_graphed(const std::string& prog, double key = 0.0) {
-    if (prog == "sweep" && prec == 2) prep_f32();
+    if (prog == "sweep" && prec == 2) prep_sweep32();
     if (!use_graph || profile) {
       run_program(prog);
       return;
@@ -2395,7 +2407,7 @@ struct ccq_hip_ctx {
   void run_sweep() { run_graphed("sweep"); }
 
   void sweep(int64_t nsec = -1) {
-    if (prec == 2) prep_f32();
+    if (prec == 2) prep_sweep32();
     run_program("sweep", nsec);
   }
 
@@ -2460,10 +2472,7 @@ struct ccq_hip_ctx {
   void run_program(const std::string& prog, int64_t nsec = -1) {
     if (!ints_ready) throw Fail("integrals not uploaded");
     run_prec = prog == "sweep" ? prec : std::min(prec, 1);
-    if (run_prec == 2) {
-      if (!f32_ready || views32_stale) throw Fail("FP32 shadows not prepared");
-      HIPCHK(ccqk::launch_cast_f32(stream, tvec32, tvec.p, dev_t));
-    }
+    if (run_prec == 2 && (!f32_ready || views32_stale)) throw Fail("FP32 shadows not prepared");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (profile) {
       HIPCHK(hipEventCreate(&e0));
@@ -2993,9 +3002,13 @@ bool solve_dense(std::vector<double> A, std::vector<double>& b, int n) {
 }
 
 template <class F>
-int guarded(ccq_hip_ctx* ctx, F&& fn) {
+// keep: the call does not write t_vec (energy, DIIS record, residuum, reads,
+// switches), so the float copy of t_vec stays valid across it
+int guarded(ccq_hip_ctx* ctx, F&& fn, bool keep = false) {
   if (!ctx) return 2;
   ++ctx->api_calls;
+  ctx->t32_valid_at_call = ctx->t32_valid;
+  if (!keep) ctx->t32_valid = false;
   try {
     fn();
     return 0;
@@ -3219,7 +3232,7 @@ int ccq_hip_get_t(ccq_hip_ctx* ctx, double* t_vec, int64_t t_size) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     // blocks not resident (T3 for CCSD) are zero, as in ccq
     for (int64_t i = ctx->dev_t; i < ctx->t_size; ++i) t_vec[i] = 0.0;
-  });
+  }, true);
 }
 
 int64_t ccq_hip_t4_size(const ccq_hip_ctx* ctx) {
@@ -3241,7 +3254,7 @@ int ccq_hip_get_t4(ccq_hip_ctx* ctx, double* t4, int64_t n) {
     HIPCHK(hipMemcpyAsync(t4, ctx->tvec.p + ctx->t_size, sizeof(double) * n, hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-  });
+  }, true);
 }
 
 // ---- ccq's amplitude files (Fortran unformatted records, restart.cpp) ----
@@ -3261,7 +3274,7 @@ int ccq_hip_write_vecs(ccq_hip_ctx* ctx, int64_t iter, int64_t diis_space, const
                                     "cc_iter_mat-" + std::to_string(slot) + ".bin"})
       if (!ccqio::write_record(d + name, t.data(), ctx->vec_size))
         throw Fail("write_vecs: cannot write " + d + name);
-  });
+  }, true);
 }
 
 int ccq_hip_read_vec(ccq_hip_ctx* ctx, const char* path) {
@@ -3293,7 +3306,7 @@ int ccq_hip_write_t4(ccq_hip_ctx* ctx, const char* bin_file) {
       if (!ccqio::write_record(path, t4.data() + off, sz)) throw Fail("write_t4: cannot write " + path);
       off += sz;
     }
-  });
+  }, true);
 }
 
 int ccq_hip_read_t4(ccq_hip_ctx* ctx, const char* bin_file) {
@@ -3314,11 +3327,14 @@ int ccq_hip_read_t4(ccq_hip_ctx* ctx, const char* bin_file) {
 }
 
 int ccq_hip_update(ccq_hip_ctx* ctx) {
-  return guarded(ctx, [&] { ctx->run_sweep(); });
+  return guarded(ctx, [&] {
+    ctx->run_sweep();
+    ctx->t32_valid = ctx->prec == 2;
+  });
 }
 
 int ccq_hip_energy(ccq_hip_ctx* ctx, double* e) {
-  return guarded(ctx, [&] { *e = ctx->energy(); });
+  return guarded(ctx, [&] { *e = ctx->energy(); }, true);
 }
 
 int ccq_hip_diis_record(ccq_hip_ctx* ctx, int64_t iter) {
@@ -3340,7 +3356,7 @@ int ccq_hip_diis_record(ccq_hip_ctx* ctx, int64_t iter) {
                                      ctx->vec_size, ctx->rpart.p, kPartialBlocks));
     ctx->rec_call = ctx->api_calls;
     ctx->rec_iter = iter;
-  });
+  }, true);
 }
 
 int ccq_hip_residuum(ccq_hip_ctx* ctx, int64_t iter, double* res) {
@@ -3354,7 +3370,7 @@ int ccq_hip_residuum(ccq_hip_ctx* ctx, int64_t iter, double* res) {
     const double* prev = iter == 1 ? nullptr : ctx->ring.p + ctx->slot_of(iter - 1) * len;
     HIPCHK(ccqk::launch_diff_sq(ctx->stream, prev, cur, len, ctx->partial.p, kPartialBlocks));
     *res = std::sqrt(std::sqrt(ctx->reduce_partials(kPartialBlocks)));
-  });
+  }, true);
 }
 
 int ccq_hip_diis_extrapolate(ccq_hip_ctx* ctx) {
@@ -3439,7 +3455,7 @@ int ccq_hip_get_l(ccq_hip_ctx* ctx, double* l_vec, int64_t l_size) {
     HIPCHK(hipMemcpyAsync(l_vec, ctx->lvec.p, sizeof(double) * l_size, hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-  });
+  }, true);
 }
 
 int ccq_hip_cct3(ccq_hip_ctx* ctx, double* e23) {
@@ -3461,14 +3477,14 @@ int ccq_hip_set_precision(ccq_hip_ctx* ctx, int64_t mode) {
     if (mode < 0 || mode > 2)
       throw Fail("precision mode must be 0 (FP64), 1 (FP32 MFMA) or 2 (FP32 MFMA, FP32 storage)");
     ctx->prec = (int)mode;
-  });
+  }, true);
 }
 
 int ccq_hip_profile(ccq_hip_ctx* ctx, int64_t enable) {
   return guarded(ctx, [&] {
     ctx->profile = enable != 0;
     ctx->prof.clear();
-  });
+  }, true);
 }
 
 int ccq_hip_profile_report(ccq_hip_ctx* ctx, char* buf, int64_t buflen) {
@@ -3480,7 +3496,7 @@ int ccq_hip_profile_report(ccq_hip_ctx* ctx, char* buf, int64_t buflen) {
     std::string s = os.str();
     if ((int64_t)s.size() + 1 > buflen) throw Fail("profile buffer too small");
     std::memcpy(buf, s.c_str(), s.size() + 1);
-  });
+  }, true);
 }
 
 int ccq_hip_mfma_ceiling(int64_t kind, int64_t blocks, int64_t iters, double* seconds, double* flops) {
@@ -3510,7 +3526,7 @@ int ccq_hip_mfma_ceiling(int64_t kind, int64_t blocks, int64_t iters, double* se
 }
 
 int ccq_hip_sync(ccq_hip_ctx* ctx) {
-  return guarded(ctx, [&] { HIPCHK(hipStreamSynchronize(ctx->stream)); });
+  return guarded(ctx, [&] { HIPCHK(hipStreamSynchronize(ctx->stream)); }, true);
 }
 
 int ccq_hip_debug_sweep(ccq_hip_ctx* ctx, int64_t nsections) {
@@ -3528,7 +3544,7 @@ int ccq_hip_debug_read(ccq_hip_ctx* ctx, const char* tensor_id, double* out, int
     if (n != total) throw Fail("debug_read size mismatch");
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, t.ptr, sizeof(double) * total, hipMemcpyDeviceToHost));
-  });
+  }, true);
 }
 
 const char* ccq_hip_last_error(const ccq_hip_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }

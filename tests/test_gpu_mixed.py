@@ -216,3 +216,37 @@ def test_fp32_storage_sweeps_from_zero(method, rhf, kw):
     scale = np.abs(out[0]).max()
     diff = np.abs(out[1] - out[0]).max()
     assert 1e-13 < diff < 1e-5 * scale, (diff, scale)
+
+
+def test_fp32_storage_copy_of_amplitudes_tracks_writes():
+    """The float copy of the amplitudes in FP32-storage sweeps is refreshed
+    only when something may have written them: a sweep after set_t sees the
+    new amplitudes, and sweeps that follow sweeps (energy, DIIS record and
+    residuum in between) equal sweeps after an explicit re-upload of the
+    same amplitudes, bit for bit."""
+    _say("case fp32 storage amplitude copy")
+    c = Case(12, 3, 24, 19, occ_b=2)
+    v1 = _random_state(c, "CCSDT", 41)
+    v2 = _random_state(c, "CCSDT", 43)
+    a = _engine(c, "CCSDT", shift=0.02)
+    b = _engine(c, "CCSDT", shift=0.02)
+    for eng in (a, b):
+        eng.set_precision(2)
+    a.set_t(v1)
+    a.update()
+    a.set_t(v2)
+    a.update()
+    b.set_t(v2)
+    b.update()
+    np.testing.assert_array_equal(a.get_t(), b.get_t())
+    for it in range(1, 4):
+        for eng, reload in ((a, False), (b, True)):
+            if reload:
+                eng.set_t(eng.get_t())
+            eng.update()
+            eng.energy()
+            eng.diis_record(it)
+            eng.residuum(it)
+        np.testing.assert_array_equal(a.get_t(), b.get_t())
+    a.close()
+    b.close()
