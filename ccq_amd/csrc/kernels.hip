@@ -215,7 +215,7 @@ __global__ void fill(double* __restrict__ x, int64_t n, double v) {
 
 // One thread per element of the full block; only canonical tuples (strictly
 // increasing inside every antisymmetric group) do work.
-template <bool SUMSQ, typename IO>
+template <bool SUMSQ, typename IO, int RANK>
 __global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __restrict__ t,
                                                      const double* __restrict__ U,
                                                      const int* __restrict__ perms,
@@ -223,8 +223,9 @@ __global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __r
                                                      const double* __restrict__ fdiag_a,
                                                      const double* __restrict__ fdiag_b,
                                                      double* __restrict__ sumsq) {
-  const int rank = P.rank;
-  const int half = rank / 2;
+  // the block's rank (P.rank) as a constant: every index loop unrolls
+  constexpr int rank = RANK;
+  constexpr int half = RANK / 2;
   double sq = 0.0;
   // one thread per canonical tuple (occupied half oc, virtual half vc; the
   // virtual half runs fastest, so the identity image is read and written
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(256) void jacobi_update(UpdateParams P, double* __r
 // The same step with one wave per canonical tuple, for blocks with many
 // images (a T4 block of one spin has 576): lane l gathers and scatters images
 // l, l + 64, ...; the image sum is a fixed butterfly over the wave.
-template <bool SUMSQ, typename IO>
+template <bool SUMSQ, typename IO, int RANK>
 __global__ __launch_bounds__(256) void jacobi_update_wave(UpdateParams P, double* __restrict__ t,
                                                           const double* __restrict__ U,
                                                           const int* __restrict__ perms,
@@ -296,8 +297,9 @@ __global__ __launch_bounds__(256) void jacobi_update_wave(UpdateParams P, double
                                                           const double* __restrict__ fdiag_a,
                                                           const double* __restrict__ fdiag_b,
                                                           double* __restrict__ sumsq) {
-  const int rank = P.rank;
-  const int half = rank / 2;
+  // the block's rank (P.rank) as a constant: every index loop unrolls
+  constexpr int rank = RANK;
+  constexpr int half = RANK / 2;
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   double sq = 0.0;
@@ -1144,28 +1146,41 @@ int update_blocks(const UpdateParams& P) {
   return P.nimg >= kWaveImages ? grid_for(P.total * 64, 256, 16384) : grid_for(P.total, 256);
 }
 
+template <typename IO, int R>
+static void launch_update_r(hipStream_t s, const UpdateParams& P, double* t, const double* U,
+                            const int* perms, const double* signs, const double* fa,
+                            const double* fb, double* sumsq) {
+  const dim3 grid(update_blocks(P)), blk(256);
+  if (P.nimg >= kWaveImages) {
+    if (sumsq)
+      hipLaunchKernelGGL((jacobi_update_wave<true, IO, R>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
+    else
+      hipLaunchKernelGGL((jacobi_update_wave<false, IO, R>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
+    return;
+  }
+  if (sumsq)
+    hipLaunchKernelGGL((jacobi_update<true, IO, R>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
+  else
+    hipLaunchKernelGGL((jacobi_update<false, IO, R>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
+}
+
 template <typename IO>
 static void launch_update_io(hipStream_t s, const UpdateParams& P, double* t, const double* U,
                              const int* perms, const double* signs, const double* fa,
                              const double* fb, double* sumsq) {
-  const dim3 grid(update_blocks(P)), blk(256);
-  if (P.nimg >= kWaveImages) {
-    if (sumsq)
-      hipLaunchKernelGGL((jacobi_update_wave<true, IO>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
-    else
-      hipLaunchKernelGGL((jacobi_update_wave<false, IO>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
-    return;
+  switch (P.rank) {
+    case 2: launch_update_r<IO, 2>(s, P, t, U, perms, signs, fa, fb, sumsq); break;
+    case 4: launch_update_r<IO, 4>(s, P, t, U, perms, signs, fa, fb, sumsq); break;
+    case 6: launch_update_r<IO, 6>(s, P, t, U, perms, signs, fa, fb, sumsq); break;
+    default: launch_update_r<IO, 8>(s, P, t, U, perms, signs, fa, fb, sumsq); break;
   }
-  if (sumsq)
-    hipLaunchKernelGGL((jacobi_update<true, IO>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
-  else
-    hipLaunchKernelGGL((jacobi_update<false, IO>), grid, blk, 0, s, P, t, U, perms, signs, fa, fb, sumsq);
 }
 
 hipError_t launch_update(hipStream_t s, const UpdateParams& P, double* t, const double* U,
                          const int* perms, const double* signs, const double* fa,
                          const double* fb, double* sumsq) {
   if (!P.coc || !P.cvc || P.n_vc <= 0) return hipErrorInvalidValue;
+  if (P.rank != 2 && P.rank != 4 && P.rank != 6 && P.rank != 8) return hipErrorInvalidValue;
   if (P.total <= 0) return hipSuccess;
   if (P.u32) launch_update_io<float>(s, P, t, U, perms, signs, fa, fb, sumsq);
   else launch_update_io<double>(s, P, t, U, perms, signs, fa, fb, sumsq);
