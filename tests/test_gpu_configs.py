@@ -15,7 +15,8 @@ go, and check what the mathematics fixes there:
   spin-orbital CCSDT oracle (oracle/ccsdt_so.py) finds every residual block
   at zero and the same energy; all stored images are antisymmetric; the
   closed-shell (rhf) sweep converges to the same energy; the mixed-precision
-  solve ends on the FP64 fixed point;
+  solves (FP32 contractions; FP32 storage with the bench's switch) end on
+  the FP64 fixed point;
 * CCSDt + CC(t;3), 80 orbitals / 7 occupied, D2h irreps (act_occ 3,
   act_unocc 8, L = 48, seed 3): symmetry-forbidden T1, T2, T3 and Q-space
   T3 stay exactly zero, the packed t_vec has get_t_sizes_act's size, and the
@@ -170,7 +171,9 @@ def test_ccsdt_n40_rhf_equals_uhf_and_mixed_equals_fp64():
     s, ints = _workload("ccsdt-n40")
     out = {}
     for label, kw, mixed in (("uhf", {}, None), ("rhf", {"rhf": True}, None),
-                             ("mixed", {}, SV.Mixed(switch_de=1e-6))):
+                             ("mixed", {}, SV.Mixed(switch_de=1e-6)),
+                             # the bench's mixed leg: rhf, FP32 storage, switch at 3e-5
+                             ("mixed2", {"rhf": True}, SV.Mixed(switch_de=3e-5, mode=2))):
         eng = _engine(s, ints, "CCSDT", **kw)
         res = SV.jacobi_iter(eng, tol=1e-11, max_iter=100, diis_space=5, out=None, mixed=mixed)
         assert res.converged, label
@@ -181,6 +184,8 @@ def test_ccsdt_n40_rhf_equals_uhf_and_mixed_equals_fp64():
     assert np.abs(out["rhf"][1] - out["uhf"][1]).max() < 1e-8
     assert abs(out["mixed"][0] - out["uhf"][0]) < 1e-9
     assert np.abs(out["mixed"][1] - out["uhf"][1]).max() < 1e-8
+    assert abs(out["mixed2"][0] - out["rhf"][0]) < 1e-9
+    assert np.abs(out["mixed2"][1] - out["rhf"][1]).max() < 1e-8
 
 
 # ------------------------------------------------------ CCSDt + CC(t;3) n=80
