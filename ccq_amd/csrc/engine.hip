@@ -1318,7 +1318,33 @@ struct ccq_hip_ctx {
                    "/" + plan_tag(op.plan) + (op.pack_k || op.t_rowC2 >= 0 || op.t_colC2 >= 0 ? "p" : "");
   }
 
+  // A dressing section (T1-dressed intermediates, name "dress...") and the
+  // compute section right after it in its program are planned as one: their
+  // products form one dependency DAG, so the later section's products that
+  // do not read the dressed intermediates can share the dressing levels.
+  // The merged section keeps the later section's name (the P-space rule
+  // keys on residual section names); consecutive residual sections, each
+  // feeding its own masked update, stay apart.  CCQ_HIP_MERGE=0 keeps the
+  // program's sections.
+  bool merge_sections = true;
+  void merge_compute_sections() {
+    std::vector<Section> out;
+    for (auto& sec : sections) {
+      if (!out.empty() && sec.kind == Section::Compute && out.back().kind == Section::Compute &&
+          out.back().prog == sec.prog && out.back().name.rfind("dress", 0) == 0) {
+        std::vector<Op> ops = std::move(out.back().ops);
+        ops.insert(ops.end(), sec.ops.begin(), sec.ops.end());
+        out.back() = std::move(sec);
+        out.back().ops = std::move(ops);
+        continue;
+      }
+      out.push_back(std::move(sec));
+    }
+    sections = std::move(out);
+  }
+
   void plan() {
+    if (merge_sections) merge_compute_sections();
     std::vector<int> host32;
     auto push32 = [&](const std::vector<int64_t>& v) {
       int64_t o = (int64_t)host32.size();
@@ -3080,6 +3106,7 @@ int ccq_hip_create(const ccq_hip_dims* dims, const ccq_hip_opts* opts, ccq_hip_c
     if (const char* g = std::getenv("CCQ_HIP_TUNE32")) ctx->tune32 = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_GROUP_MINK")) ctx->group_min_k = std::max(1, std::atoi(g));
     if (const char* g = std::getenv("CCQ_HIP_GROUP_REG")) ctx->group_reg = std::string(g) != "0";
+    if (const char* g = std::getenv("CCQ_HIP_MERGE")) ctx->merge_sections = std::string(g) != "0";
     if (const char* g = std::getenv("CCQ_HIP_STREAMS"))
       ctx->nstreams = std::max(1, std::min(ccq_hip_ctx::kMaxStreams, std::atoi(g)));
     HIPCHK(hipSetDevice(ctx->device));
